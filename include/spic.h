@@ -1,0 +1,226 @@
+/*
+ * spic.h -- C ABI of libspic.so, the MI355X (gfx950) particle hot path of
+ * sputniPIC: the implicit predictor-corrector mover and the ten-moment
+ * cloud-in-cell deposit.
+ *
+ * Every array argument is a DEVICE pointer (HBM-resident); sizes are plain
+ * integers; no C++ or framework types cross this boundary.  Every entry
+ * point returns an int status (SPIC_OK == 0) and never throws or exits;
+ * spic_last_error() gives a thread-local message for the last failure.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   spic_mover_pc       int  mover_PC(particles*, EMfield*, grid*, parameters*)
+ *                       include/Mover.h:11-12, src/Mover.cpp:4-162
+ *   spic_interp_p2g     void interpP2G(particles*, interpDensSpecies*, grid*)
+ *                       include/Mover.h:19-21, src/Mover.cpp:385-572
+ *   spic_mover_interp   int  mover_interp(particles*, EMfield*, interpDensSpecies*,
+ *                                         grid*, parameters*)
+ *                       include/Mover.h:24-26, src/Mover.cpp:575-862
+ *   spic_particle_sort  void particle_sort(parameters*, particles*, grid*)
+ *                       include/Particles.h:95-97, src/Particles.cpp:72-234
+ *   spic_zero_moments   void setZeroSpeciesDensities(interpDensSpecies*, grid*, int)
+ *                       include/InterpDensNet.h, src/InterpDensNet.cpp:40-75
+ *                       (the ten node moments; rhoc is not on this path)
+ * The reference's GPU form of the same path is move_and_interpolate /
+ * batch_update_particles (include/gpu/Particles_gpu.cuh:130-138,
+ * src/gpu/Particles_gpu.cu:221-307); this ABI supersedes it with
+ * HBM-resident particles (no host batching).
+ *
+ * The C++ layer in include/sputnipic/sputnipic.h keeps the reference's own
+ * C++ signatures (host structs) on top of these calls.
+ */
+#ifndef SPIC_H
+#define SPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPIC_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  SPIC_OK = 0,
+  SPIC_EINVAL = 1,      /* bad argument (null pointer, size, layout) */
+  SPIC_EHIP = 2,        /* HIP runtime error */
+  SPIC_EDOMAIN = 3,     /* a particle left the grid by more than one cell:
+                           the reference would index outside its arrays */
+  SPIC_EUNSUPPORTED = 4,/* a configuration the reference does not run the same
+                           way (n_sub_cycles != 1: the CPU mover has no
+                           sub-cycle loop, src/Mover.cpp:11) */
+  SPIC_ENOMEM = 5       /* device allocation failed */
+};
+
+/* Precision of particle, field and per-particle arithmetic (the
+ * reference's FPpart/FPfield/FPinterp, include/PrecisionTypes.h:6-10).
+ * Node moments on the device are always FP64. */
+enum { SPIC_F64 = 0, SPIC_F32 = 1 };
+
+/* Grid constants (struct grid, include/Grid.h:11-63).  Cell counts include
+ * the two guard cells (nxc = physical + 2) and node counts are nxc + 1, as
+ * setGrid makes them (src/Grid.cpp:9-15).  invd* / invVOL are the
+ * FPfield values of the reference (exactly representable in double).
+ * xn/yn/zn are device arrays of the node coordinates along each axis
+ * (length nxn, nyn, nzn; element type per prec): the reference's XN[i][j][k]
+ * depends on i only (src/Grid.cpp:53-61), so XN[i][*][*] == xn[i]. */
+typedef struct spic_grid {
+  int nxc, nyc, nzc;
+  int nxn, nyn, nzn;
+  double dx, dy, dz;
+  double invdx, invdy, invdz, invVOL;
+  double xStart, yStart, zStart;
+  double Lx, Ly, Lz;
+  int prec; /* SPIC_F64 or SPIC_F32 */
+  const void *xn, *yn, *zn;
+} spic_grid;
+
+/* Node-centred fields (struct EMfield, include/EMfield.h:8-25): device
+ * arrays [nxn][nyn][nzn], k fastest, element type per prec. */
+typedef struct spic_field {
+  const void *Ex, *Ey, *Ez, *Bxn, *Byn, *Bzn;
+} spic_field;
+
+/* One species (struct particles, include/Particles.h:11-56): device SoA
+ * arrays of npmax elements, element type per prec.  nop particles are
+ * moved and deposited; particle_sort covers npmax as the reference does
+ * (src/Particles.cpp:178). */
+typedef struct spic_species {
+  void *x, *y, *z, *u, *v, *w;
+  void *q;
+  int64_t nop;
+  int64_t npmax;
+  double qom;
+  int NiterMover;
+  int n_sub_cycles;
+} spic_species;
+
+/* Ten node moments of one species (struct interpDensSpecies,
+ * include/InterpDensSpecies.h:9-40): device FP64 arrays [nxn][nyn][nzn]. */
+typedef struct spic_moments {
+  double *rhon, *Jx, *Jy, *Jz, *pxx, *pxy, *pxz, *pyy, *pyz, *pzz;
+} spic_moments;
+
+/* The members of struct parameters the mover reads (include/Parameters.h):
+ * dt, c and the field periodicity flags PERIODICX/Y/Z, which the mover uses
+ * for the particle boundary conditions (src/Mover.cpp:103-157). */
+typedef struct spic_push {
+  double dt;
+  double c;
+  int PERIODICX, PERIODICY, PERIODICZ;
+} spic_push;
+
+/* ---------------------------------------------------------------------- */
+/* Reference-path entry points (caller's particle order is preserved).    */
+/* `stream` is a hipStream_t (NULL: the null stream).  In synchronous mode */
+/* (the default, spic_set_async(0)) each call waits for its stream and     */
+/* reports out-of-domain particles as SPIC_EDOMAIN; in asynchronous mode   */
+/* calls only enqueue work and spic_check_domain() reports later.          */
+/* ---------------------------------------------------------------------- */
+
+/* mover_PC: NiterMover predictor-corrector iterations per particle, final
+ * update, per-axis periodic / reflecting particle BC.  Updates x..w in
+ * place; q is read by nothing here. */
+int spic_mover_pc(const spic_grid *grd, const spic_field *field,
+                  spic_species *sp, const spic_push *push, void *stream);
+
+/* interpP2G: ADDS (+=) the ten CIC moments of sp's nop particles to mom. */
+int spic_interp_p2g(const spic_grid *grd, const spic_species *sp,
+                    spic_moments *mom, void *stream);
+
+/* mover_interp: mover_PC then interpP2G, fused per particle (13 words of
+ * particle traffic instead of 19). */
+int spic_mover_interp(const spic_grid *grd, const spic_field *field,
+                      spic_species *sp, spic_moments *mom,
+                      const spic_push *push, void *stream);
+
+/* particle_sort: stable bin sort of all 7 arrays over npmax, bins of
+ * sort_cps guard-inclusive cells per side with a trailing "lost" bin, in
+ * exactly the reference's order (src/Particles.cpp:155-233). */
+int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
+                       spic_species *sp, void *stream);
+
+/* setZeroSpeciesDensities, node-moment part: zero the ten arrays. */
+int spic_zero_moments(const spic_grid *grd, spic_moments *mom, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Synthetic Harris-sheet (GEM) input on the device, for runs too large    */
+/* for a CPU initial condition.  Restates initGEM (src/IC.cpp:38-177) and  */
+/* interpN2Crho (src/InterpDensSpecies.cpp:64-77) with one change: the     */
+/* Maxwellian draws come from a counter-based generator (SplitMix64 of     */
+/* seed, species, particle, draw) instead of glibc rand(), so particles    */
+/* are generated independently.  Positions, charges and fields follow the  */
+/* reference formulas; the sub-lattice is cell-major as the reference's.   */
+/* ---------------------------------------------------------------------- */
+typedef struct spic_gem_params {
+  double B0x, B0y, B0z, delta; /* Harris sheet (inputfiles/GEM_3D.inp) */
+  double pertX;                /* X-point perturbation amplitude; 0.4 in
+                                  src/IC.cpp:11 */
+  double fourpi;               /* 12.5663706144 (src/RW_IO.cpp:39) */
+} spic_gem_params;
+
+typedef struct spic_gem_species {
+  int species;      /* index, part of the random counter */
+  uint32_t seed;    /* run seed, part of the random counter */
+  int sheet;        /* 1: current-sheet density profile (species 0,1 of the
+                       reference); 0: uniform background */
+  int npcelx, npcely, npcelz;
+  double qom, rhoINIT;
+  double u0, v0, w0, uth, vth, wth;
+} spic_gem_species;
+
+/* Fill the six node fields (all nodes, ghosts included). */
+int spic_gem_fields(const spic_grid *grd, const spic_gem_params *gem,
+                    spic_field *out, void *stream);
+/* Fill out->npmax particles of one species: npcelx*npcely*npcely per
+ * physical cell in cell-major order (kk < npcely, src/IC.cpp:128).
+ * out->npmax must equal that count times the physical cell count. */
+int spic_gem_particles(const spic_grid *grd, const spic_gem_params *gem,
+                       const spic_gem_species *sp, spic_species *out,
+                       void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* HIP event timing on a stream (for callers without HIP headers).        */
+/* ---------------------------------------------------------------------- */
+int spic_event_create(void **event);
+int spic_event_destroy(void *event);
+int spic_event_record(void *event, void *stream);
+int spic_event_elapsed_ms(float *ms, void *start, void *end);
+
+/* ---------------------------------------------------------------------- */
+/* Modes, errors, device memory helpers (thin wrappers over HIP so callers */
+/* without HIP headers -- the C++ layer, ctypes -- can stage data).        */
+/* ---------------------------------------------------------------------- */
+const char *spic_last_error(void);
+int spic_abi_version(void);
+
+/* Synchronous (0, default) or asynchronous (1) reporting, see above. */
+int spic_set_async(int async_mode);
+/* Arithmetic mode.  0 (default): FMA contraction allowed (production).
+ * 1: no FMA contraction, every operation rounded in the reference's order
+ * and operand types, so the particle state matches the CPU reference bit
+ * for bit. */
+int spic_set_strict_fp(int strict);
+/* In asynchronous mode: wait for `stream`, return SPIC_EDOMAIN if any
+ * out-of-domain particle was seen since the last check, and clear it. */
+int spic_check_domain(void *stream);
+
+int spic_device_count(int *count);
+int spic_malloc(void **dptr, size_t bytes);
+int spic_free(void *dptr);
+int spic_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int spic_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int spic_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int spic_memset(void *dptr, int value, size_t bytes);
+int spic_synchronize(void);
+int spic_stream_create(void **stream);
+int spic_stream_destroy(void *stream);
+int spic_stream_synchronize(void *stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* SPIC_H */

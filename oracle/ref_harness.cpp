@@ -1,0 +1,294 @@
+// ref_harness.cpp -- test-only C entry points around the reference's own
+// hot-path translation units, compiled from the reference checkout by
+// oracle/Makefile into oracle/_ref/libref_{f32,f64}.so.
+//
+// TEST INFRASTRUCTURE ONLY: used to pin the C restatement in spic_oracle.c
+// and to write the fixtures under tests/golden/.  Nothing here is product
+// code.  It includes the reference headers and calls, unmodified:
+//   setGrid (src/Grid.cpp:5), field_allocate / field_aux_allocate
+//   (src/EMfield.cpp), interp_dens_species_allocate
+//   (src/InterpDensSpecies.cpp:4), particle_allocate (src/Particles.cpp:5),
+//   initGEM (src/IC.cpp:7), mover_PC / interpP2G / mover_interp
+//   (src/Mover.cpp), particle_sort (src/Particles.cpp:72), applyBCids
+//   (src/BC.cpp:362), setZeroSpeciesDensities (src/InterpDensNet.cpp:40).
+// It fills struct parameters directly, with the defaults of
+// src/RW_IO.cpp:37-295 for keys it does not set, because readInputFile
+// lives in RW_IO.cpp, which needs HDF5 (absent here).
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "BC.h"
+#include "EMfield.h"
+#include "Grid.h"
+#include "IC.h"
+#include "InterpDensNet.h"
+#include "InterpDensSpecies.h"
+#include "Mover.h"
+#include "Parameters.h"
+#include "Particles.h"
+
+template <class T>
+static void to_double(const T *src, double *dst, long n) {
+  for (long i = 0; i < n; i++) dst[i] = (double)src[i];
+}
+template <class T>
+static void from_double(const double *src, T *dst, long n) {
+  for (long i = 0; i < n; i++) dst[i] = (T)src[i];
+}
+
+extern "C" {
+
+struct ref_cfg {
+  int nxc, nyc, nzc;
+  double Lx, Ly, Lz;
+  double dt, c;
+  int NiterMover, n_sub_cycles;
+  int ns;
+  int npcelx[6], npcely[6], npcelz[6];
+  double qom[6], rhoINIT[6], uth[6], vth[6], wth[6], u0[6], v0[6], w0[6];
+  double B0x, B0y, B0z, delta;
+  int PERIODICX, PERIODICY, PERIODICZ;
+  long sort_cps;
+};
+
+struct ref_state {
+  parameters param;
+  grid grd;
+  EMfield field;
+  EMfield_aux field_aux;
+  interpDensSpecies *ids;
+  particles *part;
+};
+
+static void set_defaults(parameters *p) {
+  // src/RW_IO.cpp:37-295 defaults for the members the hot path touches
+  p->c = 1;
+  p->fourpi = 12.5663706144;
+  p->th = 1;
+  p->NiterMover = 3;
+  p->n_sub_cycles = 1;
+  p->sort = false;
+  p->sort_every_n = 50;
+  p->sort_cps = 4;
+  p->number_of_batches = 16;
+  p->threads_per_block = 256;
+  p->NpMaxNpRatio = 1.0;
+  p->PERIODICX = true;
+  p->PERIODICY = false;
+  p->PERIODICZ = true;
+  p->PERIODICX_P = true;
+  p->PERIODICY_P = false;
+  p->PERIODICZ_P = true;
+  p->delta = 0.5;
+  p->RESTART = false;
+  p->first_cycle_n = 1;
+  // a directory that does not exist keeps initGEM from writing IC dumps
+  // (src/IC.cpp:178-183)
+  p->RestartDirName = "/nonexistent-spic-ref-harness";
+  p->SaveDirName = p->RestartDirName;
+}
+
+void *ref_create(const ref_cfg *cfg) {
+  ref_state *s = new ref_state();
+  parameters *p = &s->param;
+  set_defaults(p);
+  p->nxc = cfg->nxc;
+  p->nyc = cfg->nyc;
+  p->nzc = cfg->nzc;
+  p->Lx = cfg->Lx;
+  p->Ly = cfg->Ly;
+  p->Lz = cfg->Lz;
+  p->dt = cfg->dt;
+  p->c = cfg->c;
+  p->NiterMover = cfg->NiterMover;
+  p->n_sub_cycles = cfg->n_sub_cycles;
+  p->ns = cfg->ns;
+  p->B0x = cfg->B0x;
+  p->B0y = cfg->B0y;
+  p->B0z = cfg->B0z;
+  p->delta = cfg->delta;
+  p->PERIODICX = cfg->PERIODICX != 0;
+  p->PERIODICY = cfg->PERIODICY != 0;
+  p->PERIODICZ = cfg->PERIODICZ != 0;
+  p->sort_cps = cfg->sort_cps;
+  for (int i = 0; i < p->ns; i++) {
+    p->npcelx[i] = cfg->npcelx[i];
+    p->npcely[i] = cfg->npcely[i];
+    p->npcelz[i] = cfg->npcelz[i];
+    p->qom[i] = cfg->qom[i];
+    p->rhoINIT[i] = cfg->rhoINIT[i];
+    p->uth[i] = cfg->uth[i];
+    p->vth[i] = cfg->vth[i];
+    p->wth[i] = cfg->wth[i];
+    p->u0[i] = cfg->u0[i];
+    p->v0[i] = cfg->v0[i];
+    p->w0[i] = cfg->w0[i];
+    // particle count as src/RW_IO.cpp:207-211
+    int npcel = p->npcelx[i] * p->npcely[i] * p->npcelz[i];
+    p->np[i] = (long)npcel * p->nxc * p->nyc * p->nzc;
+    p->npMax[i] = (long)(p->NpMaxNpRatio * p->np[i]);
+  }
+  setGrid(p, &s->grd);
+  field_allocate(&s->grd, &s->field);
+  field_aux_allocate(&s->grd, &s->field_aux);
+  s->ids = new interpDensSpecies[p->ns];
+  for (int is = 0; is < p->ns; is++)
+    interp_dens_species_allocate(&s->grd, &s->ids[is], is);
+  s->part = new particles[p->ns];
+  for (int is = 0; is < p->ns; is++) particle_allocate(p, &s->part[is], is);
+  // initGEM fills rhoc only on interior cells; start from zero so the
+  // guard cells hold a defined value
+  long nc = (long)s->grd.nxc * s->grd.nyc * s->grd.nzc;
+  for (int is = 0; is < p->ns; is++)
+    std::memset(s->ids[is].rhoc_flat, 0, sizeof(FPinterp) * nc);
+  initGEM(p, &s->grd, &s->field, &s->field_aux, s->part, s->ids);
+  return s;
+}
+
+void ref_destroy(void *h) {
+  ref_state *s = (ref_state *)h;
+  for (int is = 0; is < s->param.ns; is++) {
+    particle_deallocate(&s->part[is]);
+    interp_dens_species_deallocate(&s->grd, &s->ids[is]);
+  }
+  delete[] s->part;
+  delete[] s->ids;
+  field_deallocate(&s->grd, &s->field);
+  field_aux_deallocate(&s->grd, &s->field_aux);
+  grid_deallocate(&s->grd);
+  delete s;
+}
+
+int ref_fp_bytes(void) { return (int)sizeof(FPpart); }
+
+// grid scalars: nxc nyc nzc nxn nyn nzn | dx dy dz invdx invdy invdz invVOL
+// xStart yStart zStart Lx Ly Lz
+void ref_grid(void *h, int *ints, double *dbl) {
+  grid *g = &((ref_state *)h)->grd;
+  ints[0] = g->nxc; ints[1] = g->nyc; ints[2] = g->nzc;
+  ints[3] = g->nxn; ints[4] = g->nyn; ints[5] = g->nzn;
+  dbl[0] = g->dx; dbl[1] = g->dy; dbl[2] = g->dz;
+  dbl[3] = g->invdx; dbl[4] = g->invdy; dbl[5] = g->invdz;
+  dbl[6] = g->invVOL;
+  dbl[7] = g->xStart; dbl[8] = g->yStart; dbl[9] = g->zStart;
+  dbl[10] = g->Lx; dbl[11] = g->Ly; dbl[12] = g->Lz;
+}
+
+long ref_nop(void *h, int is) { return ((ref_state *)h)->part[is].nop; }
+
+
+// which: 0..6 = x y z u v w q
+void ref_get_particles(void *h, int is, int which, double *out) {
+  particles *p = &((ref_state *)h)->part[is];
+  FPpart *a[6] = {p->x, p->y, p->z, p->u, p->v, p->w};
+  if (which < 6) to_double(a[which], out, p->npmax);
+  else to_double(p->q, out, p->npmax);
+}
+void ref_set_particles(void *h, int is, int which, const double *in) {
+  particles *p = &((ref_state *)h)->part[is];
+  FPpart *a[6] = {p->x, p->y, p->z, p->u, p->v, p->w};
+  if (which < 6) from_double(in, a[which], p->npmax);
+  else from_double(in, p->q, p->npmax);
+}
+
+// which: 0..5 = Ex Ey Ez Bxn Byn Bzn ; 6..8 = XN YN ZN
+void ref_get_field(void *h, int which, double *out) {
+  ref_state *s = (ref_state *)h;
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  FPfield *a[9] = {s->field.Ex_flat,  s->field.Ey_flat,  s->field.Ez_flat,
+                   s->field.Bxn_flat, s->field.Byn_flat, s->field.Bzn_flat,
+                   s->grd.XN_flat,    s->grd.YN_flat,    s->grd.ZN_flat};
+  to_double(a[which], out, nn);
+}
+void ref_set_field(void *h, int which, const double *in) {
+  ref_state *s = (ref_state *)h;
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  FPfield *a[6] = {s->field.Ex_flat,  s->field.Ey_flat,  s->field.Ez_flat,
+                   s->field.Bxn_flat, s->field.Byn_flat, s->field.Bzn_flat};
+  from_double(in, a[which], nn);
+}
+
+// which: 0..9 = rhon Jx Jy Jz pxx pxy pxz pyy pyz pzz ; 10 = rhoc
+void ref_get_moment(void *h, int is, int which, double *out) {
+  ref_state *s = (ref_state *)h;
+  interpDensSpecies *d = &s->ids[is];
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  long nc = (long)s->grd.nxc * s->grd.nyc * s->grd.nzc;
+  FPinterp *a[10] = {d->rhon_flat, d->Jx_flat,  d->Jy_flat,  d->Jz_flat,
+                     d->pxx_flat,  d->pxy_flat, d->pxz_flat, d->pyy_flat,
+                     d->pyz_flat,  d->pzz_flat};
+  if (which < 10) to_double(a[which], out, nn);
+  else to_double(d->rhoc_flat, out, nc);
+}
+void ref_set_moment(void *h, int is, int which, const double *in) {
+  ref_state *s = (ref_state *)h;
+  interpDensSpecies *d = &s->ids[is];
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  FPinterp *a[10] = {d->rhon_flat, d->Jx_flat,  d->Jy_flat,  d->Jz_flat,
+                     d->pxx_flat,  d->pxy_flat, d->pxz_flat, d->pyy_flat,
+                     d->pyz_flat,  d->pzz_flat};
+  from_double(in, a[which], nn);
+}
+
+void ref_zero_species_moments(void *h) {
+  ref_state *s = (ref_state *)h;
+  setZeroSpeciesDensities(s->ids, &s->grd, s->param.ns);
+}
+
+int ref_mover_pc(void *h, int is) {
+  ref_state *s = (ref_state *)h;
+  return mover_PC(&s->part[is], &s->field, &s->grd, &s->param);
+}
+void ref_interp_p2g(void *h, int is) {
+  ref_state *s = (ref_state *)h;
+  interpP2G(&s->part[is], &s->ids[is], &s->grd);
+}
+int ref_mover_interp(void *h, int is) {
+  ref_state *s = (ref_state *)h;
+  return mover_interp(&s->part[is], &s->field, &s->ids[is], &s->grd, &s->param);
+}
+void ref_particle_sort(void *h, int is) {
+  ref_state *s = (ref_state *)h;
+  particle_sort(&s->param, &s->part[is], &s->grd);
+}
+void ref_apply_bc_ids(void *h, int is) {
+  ref_state *s = (ref_state *)h;
+  applyBCids(&s->ids[is], &s->grd, &s->param);
+}
+
+}  // extern "C"
+
+#include <omp.h>
+
+extern "C" {
+// The timed region of the CPU baseline: `ncycles` repetitions of the
+// reference driver's particle phase (src/sputniPIC_CPU.cpp:232-256):
+// mover_PC for each species in turn (each parallel over particles), zero
+// the species moments, then interpP2G for all species in an OpenMP loop
+// over species.  The BC fold that shares that loop in the driver is not on
+// this path and is left out.  Returns total seconds; t_mover / t_deposit
+// receive the two parts.
+double ref_time_cycles(void *h, int ncycles, double *t_mover, double *t_deposit) {
+  ref_state *s = (ref_state *)h;
+  double tm = 0, td = 0;
+  for (int c = 0; c < ncycles; c++) {
+    double t0 = omp_get_wtime();
+    for (int is = 0; is < s->param.ns; is++)
+      mover_PC(&s->part[is], &s->field, &s->grd, &s->param);
+    double t1 = omp_get_wtime();
+    setZeroSpeciesDensities(s->ids, &s->grd, s->param.ns);
+#pragma omp parallel for
+    for (int is = 0; is < s->param.ns; is++)
+      interpP2G(&s->part[is], &s->ids[is], &s->grd);
+    double t2 = omp_get_wtime();
+    tm += t1 - t0;
+    td += t2 - t1;
+  }
+  if (t_mover) *t_mover = tm;
+  if (t_deposit) *t_deposit = td;
+  return tm + td;
+}
+
+int ref_max_threads(void) { return omp_get_max_threads(); }
+}  // extern "C"

@@ -1,0 +1,20 @@
+"""sputnipic_amd -- MI355X (gfx950) implementation of sputniPIC's particle
+hot path: the implicit predictor-corrector mover (mover_PC) and the
+ten-moment CIC deposit (interpP2G), behind the reference's interface.
+
+The compute lives in ``lib/libspic.so`` (hand-written HIP, C ABI in
+``include/spic.h``); this package is the host-side mirror of the
+reference's interface over that ABI.  There is no CPU fallback.
+"""
+from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
+from .core import (DeviceArray, DeviceParticles, EMField, Event, GemParams,  # noqa: F401
+                   GemSpecies, Grid, InterpDensSpecies, Parameters, Particles, Stream,
+                   FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, device_count, gem_fields,
+                   gem_particles, harris_species, interpP2G, mover_interp, mover_PC,
+                   particle_sort, set_async, set_strict_fp, setZeroSpeciesDensities)
+
+__all__ = [
+    "SpicError", "DeviceArray", "EMField", "Grid", "InterpDensSpecies", "Parameters",
+    "Particles", "mover_PC", "interpP2G", "mover_interp", "particle_sort",
+    "setZeroSpeciesDensities", "set_strict_fp", "set_async", "device_count",
+]

@@ -1,0 +1,384 @@
+"""Host-side mirror of the reference's hot-path interface, over libspic.so.
+
+The classes hold device (HBM) copies of the reference's structures:
+
+=================  ==========================================================
+``Parameters``     the members of ``struct parameters`` the path reads
+                   (include/Parameters.h:10-235)
+``Grid``           ``struct grid`` as ``setGrid`` builds it
+                   (include/Grid.h:11-63, src/Grid.cpp:5-62)
+``EMField``        ``struct EMfield`` node fields (include/EMfield.h:8-25)
+``Particles``      ``struct particles`` SoA arrays (include/Particles.h:11-56)
+``InterpDensSpecies``  the ten node moments of ``struct interpDensSpecies``
+                   (include/InterpDensSpecies.h:9-40), FP64 on the device
+=================  ==========================================================
+
+and the module functions keep the reference's names and argument order:
+``mover_PC(part, field, grd, param)``, ``interpP2G(part, ids, grd)``,
+``mover_interp(part, field, ids, grd, param)``,
+``particle_sort(param, part, grd)`` and ``setZeroSpeciesDensities``.
+Failures raise :class:`SpicError` carrying the C status code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from ._capi import (SPIC_F32, SPIC_F64, SpicError, check, lib, spic_field, spic_grid,
+                    spic_moments, spic_push, spic_species)
+
+MOMENT_NAMES = ("rhon", "Jx", "Jy", "Jz", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz")
+FIELD_NAMES = ("Ex", "Ey", "Ez", "Bxn", "Byn", "Bzn")
+PARTICLE_NAMES = ("x", "y", "z", "u", "v", "w", "q")
+
+
+def _prec_code(prec: str) -> int:
+    if prec not in ("f64", "f32"):
+        raise ValueError("prec must be 'f64' or 'f32'")
+    return SPIC_F64 if prec == "f64" else SPIC_F32
+
+
+def _dtype(prec: str):
+    return np.float64 if prec == "f64" else np.float32
+
+
+class DeviceArray:
+    """One device allocation with numpy upload/download."""
+
+    def __init__(self, n: int, dtype, fill_zero: bool = False):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(1, self.n * self.dtype.itemsize)
+        p = C.c_void_p()
+        check(lib().spic_malloc(C.byref(p), self.nbytes), "spic_malloc")
+        self.ptr = p.value
+        if fill_zero:
+            self.zero()
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, dtype=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError(f"size {a.size} != {self.n}")
+        if self.n:
+            check(lib().spic_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes), "upload")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            check(lib().spic_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "download")
+        return out
+
+    def zero(self) -> None:
+        check(lib().spic_memset(self.ptr, 0, self.nbytes), "memset")
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().spic_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class Parameters:
+    """Members of ``struct parameters`` read on this path, with the
+    defaults of src/RW_IO.cpp:37-225."""
+    dt: float = 0.25
+    c: float = 1.0
+    NiterMover: int = 3
+    n_sub_cycles: int = 1
+    PERIODICX: bool = True
+    PERIODICY: bool = False
+    PERIODICZ: bool = True
+    sort_cps: int = 4
+
+    def push(self) -> spic_push:
+        return spic_push(self.dt, self.c, int(self.PERIODICX), int(self.PERIODICY),
+                         int(self.PERIODICZ))
+
+
+class Grid:
+    """``setGrid`` (src/Grid.cpp:5-62) for nxc x nyc x nzc physical cells on
+    an Lx x Ly x Lz box: two guard cells per axis, nodes = cells + 1, node
+    ``i`` at ``xStart + (i-1)*dx`` narrowed to FPfield, and the FPfield
+    constants invdx = (FPfield)1/dx, invVOL = (FPfield)1/(dx*dy*dz)."""
+
+    def __init__(self, nxc: int, nyc: int, nzc: int, Lx: float, Ly: float, Lz: float,
+                 prec: str = "f64"):
+        self.prec = prec
+        dt = _dtype(prec)
+        self.nxc, self.nyc, self.nzc = nxc + 2, nyc + 2, nzc + 2
+        self.nxn, self.nyn, self.nzn = self.nxc + 1, self.nyc + 1, self.nzc + 1
+        self.dx, self.dy, self.dz = Lx / nxc, Ly / nyc, Lz / nzc
+        self.invVOL = float(dt(dt(1.0) / (self.dx * self.dy * self.dz)))
+        self.invdx = float(dt(dt(1.0) / self.dx))
+        self.invdy = float(dt(dt(1.0) / self.dy))
+        self.invdz = float(dt(dt(1.0) / self.dz))
+        self.xStart = self.yStart = self.zStart = 0.0
+        self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
+        self.xn_host = np.array([self.xStart + (i - 1) * self.dx for i in range(self.nxn)],
+                                dtype=dt)
+        self.yn_host = np.array([self.yStart + (j - 1) * self.dy for j in range(self.nyn)],
+                                dtype=dt)
+        self.zn_host = np.array([self.zStart + (k - 1) * self.dz for k in range(self.nzn)],
+                                dtype=dt)
+        self._xn = DeviceArray.from_numpy(self.xn_host)
+        self._yn = DeviceArray.from_numpy(self.yn_host)
+        self._zn = DeviceArray.from_numpy(self.zn_host)
+        self.nn = self.nxn * self.nyn * self.nzn
+
+    @property
+    def shape(self):
+        return (self.nxn, self.nyn, self.nzn)
+
+    def c_struct(self) -> spic_grid:
+        return spic_grid(self.nxc, self.nyc, self.nzc, self.nxn, self.nyn, self.nzn,
+                         self.dx, self.dy, self.dz, self.invdx, self.invdy, self.invdz,
+                         self.invVOL, self.xStart, self.yStart, self.zStart,
+                         self.Lx, self.Ly, self.Lz, _prec_code(self.prec),
+                         self._xn.ptr, self._yn.ptr, self._zn.ptr)
+
+
+class EMField:
+    """Six node-centred field arrays in the grid's precision."""
+
+    def __init__(self, grd: Grid, arrays: Optional[Dict[str, np.ndarray]] = None):
+        self.grd = grd
+        dt = _dtype(grd.prec)
+        self.d = {n: DeviceArray(grd.nn, dt, fill_zero=arrays is None) for n in FIELD_NAMES}
+        if arrays is not None:
+            self.upload(arrays)
+
+    def upload(self, arrays: Dict[str, np.ndarray]) -> None:
+        for n in FIELD_NAMES:
+            self.d[n].upload(np.asarray(arrays[n]).reshape(-1))
+
+    def c_struct(self) -> spic_field:
+        return spic_field(*[self.d[n].ptr for n in FIELD_NAMES])
+
+
+class Particles:
+    """One species: SoA x, y, z, u, v, w, q of npmax elements, nop active."""
+
+    def __init__(self, arrays: Dict[str, np.ndarray], qom: float, prec: str = "f64",
+                 NiterMover: int = 3, n_sub_cycles: int = 1, nop: Optional[int] = None):
+        dt = _dtype(prec)
+        self.prec = prec
+        self.npmax = int(len(arrays["x"]))
+        self.nop = self.npmax if nop is None else int(nop)
+        self.qom = float(qom)
+        self.NiterMover = NiterMover
+        self.n_sub_cycles = n_sub_cycles
+        self.d = {k: DeviceArray.from_numpy(np.asarray(arrays[k]), dt) for k in PARTICLE_NAMES}
+
+    def download(self) -> Dict[str, np.ndarray]:
+        return {k: self.d[k].download() for k in PARTICLE_NAMES}
+
+    def c_struct(self) -> spic_species:
+        return spic_species(*[self.d[k].ptr for k in PARTICLE_NAMES], self.nop, self.npmax,
+                            self.qom, self.NiterMover, self.n_sub_cycles)
+
+
+class InterpDensSpecies:
+    """The ten node moments of one species, FP64 on the device."""
+
+    def __init__(self, grd: Grid):
+        self.grd = grd
+        self.d = {n: DeviceArray(grd.nn, np.float64, fill_zero=True) for n in MOMENT_NAMES}
+
+    def download(self) -> Dict[str, np.ndarray]:
+        return {n: self.d[n].download() for n in MOMENT_NAMES}
+
+    def c_struct(self) -> spic_moments:
+        D = C.POINTER(C.c_double)
+        return spic_moments(*[C.cast(self.d[n].ptr, D) for n in MOMENT_NAMES])
+
+
+def _check_prec(*objs):
+    precs = {o.prec for o in objs if hasattr(o, "prec")}
+    if len(precs) > 1:
+        raise ValueError(f"mixed precisions {precs}")
+
+
+def set_strict_fp(strict: bool) -> None:
+    """Reference rounding order (True) or production arithmetic (False)."""
+    check(lib().spic_set_strict_fp(int(bool(strict))), "spic_set_strict_fp")
+
+
+def set_async(async_mode: bool) -> None:
+    check(lib().spic_set_async(int(bool(async_mode))), "spic_set_async")
+
+
+def mover_PC(part: Particles, field: EMField, grd: Grid, param: Parameters) -> int:
+    """mover_PC (src/Mover.cpp:4-162); returns 0 like the reference."""
+    _check_prec(part, grd)
+    g, f, s, p = grd.c_struct(), field.c_struct(), part.c_struct(), param.push()
+    check(lib().spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), C.byref(p), None),
+          "mover_PC")
+    return 0
+
+
+def interpP2G(part: Particles, ids: InterpDensSpecies, grd: Grid) -> None:
+    """interpP2G (src/Mover.cpp:385-572): adds this species' moments."""
+    _check_prec(part, grd)
+    g, s, m = grd.c_struct(), part.c_struct(), ids.c_struct()
+    check(lib().spic_interp_p2g(C.byref(g), C.byref(s), C.byref(m), None), "interpP2G")
+
+
+def mover_interp(part: Particles, field: EMField, ids: InterpDensSpecies, grd: Grid,
+                 param: Parameters) -> int:
+    """mover_interp (src/Mover.cpp:575-862): fused push + deposit."""
+    _check_prec(part, grd)
+    g, f, s, m, p = (grd.c_struct(), field.c_struct(), part.c_struct(), ids.c_struct(),
+                     param.push())
+    check(lib().spic_mover_interp(C.byref(g), C.byref(f), C.byref(s), C.byref(m),
+                                  C.byref(p), None), "mover_interp")
+    return 0
+
+
+def particle_sort(param: Parameters, part: Particles, grd: Grid) -> None:
+    """particle_sort (src/Particles.cpp:72-234), reference output order."""
+    _check_prec(part, grd)
+    g, s = grd.c_struct(), part.c_struct()
+    check(lib().spic_particle_sort(C.byref(g), int(param.sort_cps), C.byref(s), None),
+          "particle_sort")
+
+
+def setZeroSpeciesDensities(ids: Sequence[InterpDensSpecies], grd: Grid, ns: int) -> None:
+    """setZeroSpeciesDensities (src/InterpDensNet.cpp:40-75), node moments."""
+    g = grd.c_struct()
+    for is_ in range(ns):
+        m = ids[is_].c_struct()
+        check(lib().spic_zero_moments(C.byref(g), C.byref(m), None), "zero moments")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().spic_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+# --------------------------------------------------------------------------
+# Synthetic Harris-sheet input on the device (spic_gem_fields /
+# spic_gem_particles): initGEM's formulas, counter-based Maxwellian draws.
+# --------------------------------------------------------------------------
+@dataclass
+class GemParams:
+    """Harris-sheet constants (inputfiles/GEM_3D.inp; pertX of src/IC.cpp:11)."""
+    B0x: float = 0.0195
+    B0y: float = 0.0
+    B0z: float = 0.0
+    delta: float = 0.5
+    pertX: float = 0.4
+    fourpi: float = 12.5663706144
+
+    def c_struct(self):
+        from ._capi import spic_gem_params
+        return spic_gem_params(self.B0x, self.B0y, self.B0z, self.delta, self.pertX,
+                               self.fourpi)
+
+
+@dataclass
+class GemSpecies:
+    """One species of the generator; ``sheet`` marks the current-sheet
+    density profile (species 0 and 1 of the reference IC)."""
+    qom: float
+    rhoINIT: float
+    npcelx: int = 3
+    npcely: int = 3
+    npcelz: int = 3
+    uth: float = 0.045
+    vth: float = 0.045
+    wth: float = 0.045
+    u0: float = 0.0
+    v0: float = 0.0
+    w0: float = 0.0
+    sheet: bool = True
+
+    def count(self, grd: Grid) -> int:
+        cells = (grd.nxc - 2) * (grd.nyc - 2) * (grd.nzc - 2)
+        return cells * self.npcelx * self.npcely * self.npcely
+
+
+def harris_species(ppc=(3, 3, 3)):
+    """The four species of inputfiles/GEM_3D.inp."""
+    px, py, pz = ppc
+    return [
+        GemSpecies(-64.0, 1.0, px, py, pz, 0.045, 0.045, 0.045, 0, 0, 0.0065, True),
+        GemSpecies(1.0, 1.0, px, py, pz, 0.0126, 0.0126, 0.0126, 0, 0, -0.0325, True),
+        GemSpecies(-64.0, 0.1, px, py, pz, 0.045, 0.045, 0.045, 0, 0, 0.0, False),
+        GemSpecies(1.0, 0.1, px, py, pz, 0.0126, 0.0126, 0.0126, 0, 0, 0.0, False),
+    ]
+
+
+def gem_fields(grd: Grid, gem: GemParams) -> EMField:
+    from ._capi import spic_gem_params  # noqa: F401
+    field = EMField(grd)
+    g, f, p = grd.c_struct(), field.c_struct(), gem.c_struct()
+    check(lib().spic_gem_fields(C.byref(g), C.byref(p), C.byref(f), None), "spic_gem_fields")
+    return field
+
+
+class DeviceParticles(Particles):
+    """Particles allocated on the device without a host copy."""
+
+    def __init__(self, n: int, qom: float, prec: str = "f64", NiterMover: int = 3,
+                 n_sub_cycles: int = 1):
+        dt = _dtype(prec)
+        self.prec = prec
+        self.npmax = self.nop = int(n)
+        self.qom = float(qom)
+        self.NiterMover = NiterMover
+        self.n_sub_cycles = n_sub_cycles
+        self.d = {k: DeviceArray(self.npmax, dt) for k in PARTICLE_NAMES}
+
+
+def gem_particles(grd: Grid, gem: GemParams, sp: GemSpecies, species: int, seed: int = 0,
+                  NiterMover: int = 3) -> DeviceParticles:
+    from ._capi import spic_gem_species
+    part = DeviceParticles(sp.count(grd), sp.qom, grd.prec, NiterMover)
+    c = spic_gem_species(species, seed, int(sp.sheet), sp.npcelx, sp.npcely, sp.npcelz,
+                         sp.qom, sp.rhoINIT, sp.u0, sp.v0, sp.w0, sp.uth, sp.vth, sp.wth)
+    g, p, s = grd.c_struct(), gem.c_struct(), part.c_struct()
+    check(lib().spic_gem_particles(C.byref(g), C.byref(p), C.byref(c), C.byref(s), None),
+          "spic_gem_particles")
+    return part
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().spic_stream_create(C.byref(p)), "stream create")
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(lib().spic_stream_synchronize(self.ptr), "stream sync")
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().spic_event_create(C.byref(p)), "event create")
+        self.ptr = p.value
+
+    def record(self, stream: Optional[Stream] = None):
+        check(lib().spic_event_record(self.ptr, stream.ptr if stream else None), "event record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().spic_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr), "event elapsed")
+        return ms.value
