@@ -44,11 +44,12 @@ def algorithmic_bytes(n_particles: int, nn: int, w: int, n_species: int) -> int:
     return n_particles * 13 * w + 6 * nn * w + 10 * n_species * nn * 8
 
 
-def run_form(prec: str, steps: int, warmup: int) -> dict:
+def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
+             tile) -> dict:
     import ctypes as C
 
     import sputnipic_amd as S
-    from sputnipic_amd._capi import check, lib
+    from sputnipic_amd._capi import check, lib, spic_moments
 
     L = lib()
     w = 8 if prec == "f64" else 4
@@ -59,38 +60,78 @@ def run_form(prec: str, steps: int, warmup: int) -> dict:
     parts = [S.gem_particles(grd, gem, sp, s, seed=0) for s, sp in enumerate(species)]
     moms = [S.InterpDensSpecies(grd) for _ in range(NS)]
     n_total = sum(p.npmax for p in parts)
-    log(f"{prec}: {n_total} particles resident, grid {grd.shape}")
+    counts = [p.npmax for p in parts]
+    log(f"{prec}: {n_total} particles generated, grid {grd.shape}, engine {engine}")
     param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
     stream = S.Stream()
     g_c = grd.c_struct()
     f_c = field.c_struct()
     p_c = param.push()
-    s_c = [p.c_struct() for p in parts]
     m_c = [m.c_struct() for m in moms]
-    ev = [[S.Event(), S.Event()] for _ in range(NS)]
+    m_arr = (spic_moments * NS)(*m_c)
+    eng = None
+    if engine == "tiled":
+        eng = S.Engine(grd, NS, tile=tile, sort_every=0)
+        t0 = time.perf_counter()
+        for s, p in enumerate(parts):
+            eng.load(s, p, stream)
+        stream.synchronize()
+        log(f"{prec}: loaded + tile-sorted in {time.perf_counter() - t0:.2f} s")
+        for p in parts:
+            for d in p.d.values():
+                d.free()
+        parts = None
+        kernel = f"k_tiled_phase<{prec}, {tile[0]}x{tile[1]}x{tile[2]}> (all species, one launch)"
+    else:
+        s_c = [p.c_struct() for p in parts]
+        kernel = "k_mover_deposit (plain fused, one launch per species)"
     S.set_async(True)
-    kernel_ms = []
+    marks = []  # (kind, start event, end event) of the timed cycles
+    pool = [S.Event() for _ in range(2 * (steps + 1) * (NS + 1))]
+    state = {"phase": 0, "ev": 0}
+
+    def pair():
+        i = state["ev"]
+        state["ev"] = i + 2
+        return pool[i % len(pool)], pool[(i + 1) % len(pool)]
 
     def cycle(timed: bool):
         for s in range(NS):
             check(L.spic_zero_moments(C.byref(g_c), C.byref(m_c[s]), stream.ptr), "zero")
-        for s in range(NS):
+        if eng is not None:
+            due = sort_every > 0 and state["phase"] > 0 and state["phase"] % sort_every == 0
+            if due:
+                a, b = pair()
+                a.record(stream)
+                eng.sort(stream)
+                b.record(stream)
+                if timed:
+                    marks.append(("sort", a, b))
+            a, b = pair()
+            a.record(stream)
+            check(L.spic_engine_phase(eng.ptr, C.byref(f_c), m_arr, C.byref(p_c), stream.ptr),
+                  "engine phase")
+            b.record(stream)
             if timed:
-                ev[s][0].record(stream)
-            check(L.spic_mover_interp(C.byref(g_c), C.byref(f_c), C.byref(s_c[s]),
-                                      C.byref(m_c[s]), C.byref(p_c), stream.ptr),
-                  "mover_interp")
-            if timed:
-                ev[s][1].record(stream)
-        if timed:
+                marks.append(("kernel", a, b))
+            state["phase"] += 1
+        else:
             for s in range(NS):
-                kernel_ms.append((s, ev[s][0].elapsed_ms(ev[s][1])))
+                a, b = pair()
+                a.record(stream)
+                check(L.spic_mover_interp(C.byref(g_c), C.byref(f_c), C.byref(s_c[s]),
+                                          C.byref(m_c[s]), C.byref(p_c), stream.ptr),
+                      "mover_interp")
+                b.record(stream)
+                if timed:
+                    marks.append(("kernel", a, b))
 
     try:
         for _ in range(warmup):
             cycle(False)
         stream.synchronize()
         check(L.spic_check_domain(stream.ptr), "domain check after warmup")
+        state["ev"] = 0
         t0 = time.perf_counter()
         for _ in range(steps):
             cycle(True)
@@ -100,33 +141,59 @@ def run_form(prec: str, steps: int, warmup: int) -> dict:
     finally:
         S.set_async(False)
     wall = t1 - t0
+    kernel_ms = [a.elapsed_ms(b) for k, a, b in marks if k == "kernel"]
+    sort_ms = [a.elapsed_ms(b) for k, a, b in marks if k == "sort"]
     # charge conservation on the last cycle: sum(rhon)/invVOL == sum(q)
     q_err = 0.0
+    q_tot = [0.0] * NS
+    if eng is not None:
+        # charges are constant: regenerate the per-species totals from the IC
+        tmp = [S.gem_particles(grd, gem, sp, s, seed=0) for s, sp in enumerate(species)]
+        for s in range(NS):
+            q_tot[s] = (float(tmp[s].d["q"].download().astype("float64").sum()),
+                        float(abs(tmp[s].d["q"].download().astype("float64")).sum()))
+        del tmp
+    else:
+        for s in range(NS):
+            q = parts[s].d["q"].download().astype("float64")
+            q_tot[s] = (float(q.sum()), float(abs(q).sum()))
     for s in range(NS):
         rho = moms[s].d["rhon"].download()
-        q = parts[s].d["q"].download().astype("float64")
         tot = float(rho.sum()) / grd.invVOL
-        q_err = max(q_err, abs(tot - float(q.sum())) / float(abs(q).sum()))
-    avg_kernel_ms = sum(t for _, t in kernel_ms) / len(kernel_ms)
-    per_launch_bytes = sum(algorithmic_bytes(p.npmax, grd.nn, w, 1) for p in parts) / NS
+        q_err = max(q_err, abs(tot - q_tot[s][0]) / q_tot[s][1])
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+    if eng is not None:
+        per_launch_bytes = algorithmic_bytes(n_total, grd.nn, w, NS)
+    else:
+        per_launch_bytes = sum(algorithmic_bytes(n, grd.nn, w, 1) for n in counts) / NS
     achieved = per_launch_bytes / (avg_kernel_ms * 1e-3) / 1e9
     value = n_total * steps / wall
     log(f"{prec}: {value / 1e9:.3f} G particles/s, {1e3 * wall / steps:.3f} ms/cycle, "
-        f"kernel {avg_kernel_ms:.3f} ms avg, {achieved:.1f} GB/s, charge err {q_err:.1e}")
+        f"kernel {avg_kernel_ms:.3f} ms avg, {achieved:.1f} GB/s "
+        f"({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak), sorts {len(sort_ms)} "
+        f"({(sum(sort_ms) / len(sort_ms)) if sort_ms else 0:.2f} ms each), "
+        f"charge err {q_err:.1e}")
     out = {
         "value": value,
         "ms_per_step": 1e3 * wall / steps,
         "particles": n_total,
-        "kernel": "mover_interp (plain fused, one launch per species)",
+        "kernel": kernel,
         "avg_kernel_ms": avg_kernel_ms,
         "bytes_per_launch": per_launch_bytes,
         "achieved_GBs": achieved,
         "roofline_frac": achieved / HBM_PEAK_GBS,
+        "sorts_timed": len(sort_ms),
+        "avg_sort_ms": (sum(sort_ms) / len(sort_ms)) if sort_ms else None,
+        "sort_every": sort_every if eng is not None else None,
         "charge_rel_err": q_err,
     }
-    for p in parts:
-        for d in p.d.values():
-            d.free()
+    if eng is not None:
+        out["engine_device_bytes"] = eng.stats()["device_bytes"]
+        eng.close()
+    else:
+        for p in parts:
+            for d in p.d.values():
+                d.free()
     return out
 
 
@@ -177,6 +244,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=10)
     ap.add_argument("--forms", default="f64,f32")
+    ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
+    ap.add_argument("--sort-every", type=int, default=10)
+    ap.add_argument("--tile", default="4x4x4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus}: this build measures one GPU only; the path "
@@ -192,7 +262,9 @@ def main() -> int:
         return 1
     forms = {}
     for prec in args.forms.split(","):
-        forms[prec] = run_form(prec, args.steps, args.warmup)
+        tile = tuple(int(t) for t in args.tile.split("x"))
+        forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, args.sort_every,
+                               tile)
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
     cpu = None

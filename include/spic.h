@@ -146,6 +146,65 @@ int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
 int spic_zero_moments(const spic_grid *grd, spic_moments *mom, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Tile-resident engine: the whole particle phase of a cycle -- mover_PC   */
+/* then interpP2G for every species, the loop of                          */
+/* src/sputniPIC_CPU.cpp:232-256 -- in one launch over tile-sorted,        */
+/* HBM-resident particles.  The engine owns its particle storage (two      */
+/* buffer sets, for the sort) and a particle id per slot, so particles can */
+/* be copied out in the order they were loaded.  Internally particles are  */
+/* re-sorted by tile every `sort_every` phases; the sort changes only the  */
+/* summation order of the moments, never a particle's trajectory.          */
+/* ---------------------------------------------------------------------- */
+typedef struct spic_engine spic_engine;
+
+typedef struct spic_engine_opts {
+  int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s), 8x4x4
+                                 or 2x2x2 */
+  int sort_every;             /* re-sort every N phases; <= 0: only on load */
+  int lds_partials;           /* precision of the per-tile LDS moment partials
+                                 of an FP32 engine: 0 = default (FP64),
+                                 1 = FP32, 2 = FP64.  An FP64 engine always
+                                 uses FP64. */
+} spic_engine_opts;
+
+typedef struct spic_engine_stats {
+  int64_t phases;       /* phases run */
+  int64_t sorts;        /* species sorts run */
+  int ntiles;           /* tiles of the grid */
+  int tile_x, tile_y, tile_z;
+  size_t device_bytes;  /* device memory held by the engine */
+  /* cumulative over all phases: predictor-corrector stencil gathers and
+   * particle deposits served from the LDS-staged tile, and those that fell
+   * outside it (particle more than one cell out of its tile since the last
+   * sort) and went to global memory */
+  int64_t lds_gathers, global_gathers;
+  int64_t lds_deposits, global_deposits;
+} spic_engine_stats;
+
+/* Create an engine for `ns` species (1..8) on `grd` (its node tables are
+ * copied).  opts may be NULL. */
+int spic_engine_create(spic_engine **eng, const spic_grid *grd, int ns,
+                       const spic_engine_opts *opts);
+int spic_engine_destroy(spic_engine *eng);
+/* Copy species `is` in (device arrays, caller order; nop particles, qom,
+ * NiterMover, n_sub_cycles) and sort it into tile order. */
+int spic_engine_load(spic_engine *eng, int is, const spic_species *sp,
+                     void *stream);
+/* Copy species `is` out into sp's arrays in the order it was loaded
+ * (sp->nop must equal the loaded count). */
+int spic_engine_store(spic_engine *eng, int is, spic_species *sp,
+                      void *stream);
+/* One particle phase: for every species, mover_PC then interpP2G, ADDING
+ * the moments into mom[is] (ns moment sets; zero them first as the
+ * reference driver does, src/sputniPIC_CPU.cpp:247). */
+int spic_engine_phase(spic_engine *eng, const spic_field *field,
+                      const spic_moments *mom, const spic_push *push,
+                      void *stream);
+/* Re-sort every species now. */
+int spic_engine_sort(spic_engine *eng, void *stream);
+int spic_engine_get_stats(const spic_engine *eng, spic_engine_stats *st);
+
+/* ---------------------------------------------------------------------- */
 /* Synthetic Harris-sheet (GEM) input on the device, for runs too large    */
 /* for a CPU initial condition.  Restates initGEM (src/IC.cpp:38-177) and  */
 /* interpN2Crho (src/InterpDensSpecies.cpp:64-77) with one change: the     */

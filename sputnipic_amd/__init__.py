@@ -7,7 +7,7 @@ The compute lives in ``lib/libspic.so`` (hand-written HIP, C ABI in
 reference's interface over that ABI.  There is no CPU fallback.
 """
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
-from .core import (DeviceArray, DeviceParticles, EMField, Event, GemParams,  # noqa: F401
+from .core import (DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams,  # noqa: F401
                    GemSpecies, Grid, InterpDensSpecies, Parameters, Particles, Stream,
                    FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, device_count, gem_fields,
                    gem_particles, harris_species, interpP2G, mover_interp, mover_PC,

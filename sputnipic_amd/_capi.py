@@ -80,6 +80,19 @@ class spic_gem_species(C.Structure):
                 ("uth", C.c_double), ("vth", C.c_double), ("wth", C.c_double)]
 
 
+class spic_engine_opts(C.Structure):
+    _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
+                ("sort_every", C.c_int), ("lds_partials", C.c_int)]
+
+
+class spic_engine_stats(C.Structure):
+    _fields_ = [("phases", C.c_int64), ("sorts", C.c_int64), ("ntiles", C.c_int),
+                ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
+                ("device_bytes", C.c_size_t),
+                ("lds_gathers", C.c_int64), ("global_gathers", C.c_int64),
+                ("lds_deposits", C.c_int64), ("global_deposits", C.c_int64)]
+
+
 # name -> (restype, argtypes); every entry point of include/spic.h
 SIGNATURES = {
     "spic_mover_pc": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_field),
@@ -92,6 +105,15 @@ SIGNATURES = {
     "spic_particle_sort": (C.c_int, [C.POINTER(spic_grid), C.c_int64,
                                      C.POINTER(spic_species), _P]),
     "spic_zero_moments": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments), _P]),
+    "spic_engine_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_grid), C.c_int,
+                                     C.POINTER(spic_engine_opts)]),
+    "spic_engine_destroy": (C.c_int, [_P]),
+    "spic_engine_load": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
+    "spic_engine_store": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
+    "spic_engine_phase": (C.c_int, [_P, C.POINTER(spic_field), C.POINTER(spic_moments),
+                                    C.POINTER(spic_push), _P]),
+    "spic_engine_sort": (C.c_int, [_P, _P]),
+    "spic_engine_get_stats": (C.c_int, [_P, C.POINTER(spic_engine_stats)]),
     "spic_gem_fields": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_gem_params),
                                   C.POINTER(spic_field), _P]),
     "spic_gem_particles": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_gem_params),

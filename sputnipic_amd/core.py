@@ -382,3 +382,58 @@ class Event:
         ms = C.c_float()
         check(lib().spic_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr), "event elapsed")
         return ms.value
+
+
+class Engine:
+    """Tile-resident engine (spic_engine_*): every species' mover_PC +
+    interpP2G in one launch over HBM-resident, tile-sorted particles."""
+
+    def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
+                 lds_partials: int = 0):
+        from ._capi import spic_engine_opts
+        self.grd = grd
+        self.ns = ns
+        p = C.c_void_p()
+        opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials)
+        g = grd.c_struct()
+        check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
+              "spic_engine_create")
+        self.ptr = p.value
+
+    def load(self, is_: int, part: Particles, stream: Optional["Stream"] = None) -> None:
+        s = part.c_struct()
+        check(lib().spic_engine_load(self.ptr, is_, C.byref(s), stream.ptr if stream else None),
+              "spic_engine_load")
+
+    def store(self, is_: int, part: Particles, stream: Optional["Stream"] = None) -> None:
+        s = part.c_struct()
+        check(lib().spic_engine_store(self.ptr, is_, C.byref(s),
+                                      stream.ptr if stream else None), "spic_engine_store")
+
+    def phase(self, field: EMField, ids: Sequence[InterpDensSpecies], param: Parameters,
+              stream: Optional["Stream"] = None) -> None:
+        moms = (spic_moments * self.ns)(*[ids[i].c_struct() for i in range(self.ns)])
+        f, p = field.c_struct(), param.push()
+        check(lib().spic_engine_phase(self.ptr, C.byref(f), moms, C.byref(p),
+                                      stream.ptr if stream else None), "spic_engine_phase")
+
+    def sort(self, stream: Optional["Stream"] = None) -> None:
+        check(lib().spic_engine_sort(self.ptr, stream.ptr if stream else None),
+              "spic_engine_sort")
+
+    def stats(self) -> dict:
+        from ._capi import spic_engine_stats
+        st = spic_engine_stats()
+        check(lib().spic_engine_get_stats(self.ptr, C.byref(st)), "stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().spic_engine_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

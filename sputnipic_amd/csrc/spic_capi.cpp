@@ -42,7 +42,15 @@ std::atomic<int> g_async{0};
 std::atomic<int> g_strict{0};
 using spic::fail;
 using spic::hip_fail;
+using spic::kgrid;
+using spic::kfield;
+using spic::kpart;
+using spic::kmom;
+using spic::kpush;
 
+}  // namespace
+
+namespace spic {
 // Per-thread, per-device status words: [0] mover iterates out of the grid,
 // [1] deposit cells out of the grid.  Thread-local so that concurrent
 // callers (the reference runs interpP2G from an OpenMP team, one thread per
@@ -68,6 +76,10 @@ unsigned int *status_buffer(hipError_t *err) {
   t_status.push_back(sb);
   return sb.d;
 }
+}  // namespace spic
+
+namespace {
+using spic::status_buffer;
 
 // Workspace for the device sort, one per device, grown on demand.
 std::mutex g_ws_mutex;
@@ -103,6 +115,9 @@ hipError_t workspace(size_t bytes, void **out) {
   return hipSuccess;
 }
 
+}  // namespace
+
+namespace spic {
 int check_grid(const spic_grid *g) {
   if (!g) return fail(SPIC_EINVAL, "grid is NULL");
   if (g->prec != SPIC_F64 && g->prec != SPIC_F32)
@@ -160,58 +175,20 @@ int check_push(const spic_push *p) {
   if (!(p->c != 0)) return fail(SPIC_EINVAL, "c must be nonzero");
   return SPIC_OK;
 }
+}  // namespace spic
 
-template <typename T>
-spic::KGrid<T> kgrid(const spic_grid *g) {
-  spic::KGrid<T> k;
-  k.nxn = g->nxn; k.nyn = g->nyn; k.nzn = g->nzn;
-  k.xStart = g->xStart; k.yStart = g->yStart; k.zStart = g->zStart;
-  k.Lx = g->Lx; k.Ly = g->Ly; k.Lz = g->Lz;
-  k.invdx = (T)g->invdx; k.invdy = (T)g->invdy; k.invdz = (T)g->invdz;
-  k.invVOL = (T)g->invVOL;
-  k.xn = (const T *)g->xn; k.yn = (const T *)g->yn; k.zn = (const T *)g->zn;
-  return k;
-}
+namespace {
+using spic::check_grid;
+using spic::check_species;
+using spic::check_field;
+using spic::check_moments;
+using spic::check_push;
 
-template <typename T>
-spic::KField<T> kfield(const spic_field *f) {
-  return spic::KField<T>{(const T *)f->Ex, (const T *)f->Ey, (const T *)f->Ez,
-                         (const T *)f->Bxn, (const T *)f->Byn,
-                         (const T *)f->Bzn};
-}
+}  // namespace
 
-template <typename T>
-spic::KPart<T> kpart(const spic_species *sp) {
-  return spic::KPart<T>{(T *)sp->x, (T *)sp->y, (T *)sp->z, (T *)sp->u,
-                        (T *)sp->v, (T *)sp->w, (const T *)sp->q,
-                        (int)sp->nop};
-}
-
-spic::KMom kmom(const spic_moments *m) {
-  spic::KMom k;
-  k.m[0] = m->rhon; k.m[1] = m->Jx; k.m[2] = m->Jy; k.m[3] = m->Jz;
-  k.m[4] = m->pxx; k.m[5] = m->pxy; k.m[6] = m->pxz;
-  k.m[7] = m->pyy; k.m[8] = m->pyz; k.m[9] = m->pzz;
-  return k;
-}
-
-// Push constants in the reference's types (src/Mover.cpp:11-12):
-//   FPpart dt_sub = (FPpart)dt / ((double)n_sub_cycles);
-//   FPpart dto2 = .5 * dt_sub, qomdt2 = qom * dto2 / c;   (qom is FPpart)
-template <typename T>
-spic::KPush<T> kpush(const spic_species *sp, const spic_push *p) {
-  spic::KPush<T> k;
-  const T qom = (T)sp->qom;
-  k.dt_sub = (T)((double)(T)p->dt / ((double)sp->n_sub_cycles));
-  k.dto2 = (T)(.5 * (double)k.dt_sub);
-  const T qd = qom * k.dto2;  // FPpart * FPpart
-  k.qomdt2 = (T)((double)qd / p->c);
-  k.niter = sp->NiterMover;
-  k.perx = p->PERIODICX != 0;
-  k.pery = p->PERIODICY != 0;
-  k.perz = p->PERIODICZ != 0;
-  return k;
-}
+namespace spic {
+bool async_mode() { return g_async.load() != 0; }
+bool strict_mode() { return g_strict.load() != 0; }
 
 // Prepare the status words for one call; returns the device pointer.
 int begin_call(hipStream_t s, unsigned int **st) {
@@ -250,7 +227,12 @@ int end_call(hipStream_t s, unsigned int *st, hipError_t launch) {
   return read_status(s, st, false);
 }
 
-}  // namespace
+}  // namespace spic
+
+using spic::begin_call;
+using spic::end_call;
+using spic::read_status;
+using spic::status_buffer;
 
 extern "C" {
 

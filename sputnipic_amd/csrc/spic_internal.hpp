@@ -87,6 +87,71 @@ hipError_t gem_particles(const spic_grid &g, const spic_gem_species &sp,
                          const double *q_row_dev, spic_species &out,
                          hipStream_t s);
 
+template <typename T>
+inline KGrid<T> kgrid(const spic_grid *g) {
+  KGrid<T> k;
+  k.nxn = g->nxn; k.nyn = g->nyn; k.nzn = g->nzn;
+  k.xStart = g->xStart; k.yStart = g->yStart; k.zStart = g->zStart;
+  k.Lx = g->Lx; k.Ly = g->Ly; k.Lz = g->Lz;
+  k.invdx = (T)g->invdx; k.invdy = (T)g->invdy; k.invdz = (T)g->invdz;
+  k.invVOL = (T)g->invVOL;
+  k.xn = (const T *)g->xn; k.yn = (const T *)g->yn; k.zn = (const T *)g->zn;
+  return k;
+}
+
+template <typename T>
+inline KField<T> kfield(const spic_field *f) {
+  return KField<T>{(const T *)f->Ex, (const T *)f->Ey, (const T *)f->Ez,
+                         (const T *)f->Bxn, (const T *)f->Byn,
+                         (const T *)f->Bzn};
+}
+
+template <typename T>
+inline KPart<T> kpart(const spic_species *sp) {
+  return KPart<T>{(T *)sp->x, (T *)sp->y, (T *)sp->z, (T *)sp->u,
+                        (T *)sp->v, (T *)sp->w, (const T *)sp->q,
+                        (int)sp->nop};
+}
+
+inline KMom kmom(const spic_moments *m) {
+  KMom k;
+  k.m[0] = m->rhon; k.m[1] = m->Jx; k.m[2] = m->Jy; k.m[3] = m->Jz;
+  k.m[4] = m->pxx; k.m[5] = m->pxy; k.m[6] = m->pxz;
+  k.m[7] = m->pyy; k.m[8] = m->pyz; k.m[9] = m->pzz;
+  return k;
+}
+
+// Push constants in the reference's types (src/Mover.cpp:11-12):
+//   FPpart dt_sub = (FPpart)dt / ((double)n_sub_cycles);
+//   FPpart dto2 = .5 * dt_sub, qomdt2 = qom * dto2 / c;   (qom is FPpart)
+template <typename T>
+inline KPush<T> kpush(const spic_species *sp, const spic_push *p) {
+  KPush<T> k;
+  const T qom = (T)sp->qom;
+  k.dt_sub = (T)((double)(T)p->dt / ((double)sp->n_sub_cycles));
+  k.dto2 = (T)(.5 * (double)k.dt_sub);
+  const T qd = qom * k.dto2;  // FPpart * FPpart
+  k.qomdt2 = (T)((double)qd / p->c);
+  k.niter = sp->NiterMover;
+  k.perx = p->PERIODICX != 0;
+  k.pery = p->PERIODICY != 0;
+  k.perz = p->PERIODICZ != 0;
+  return k;
+}
+
+// Call protocol shared by the entry points (spic_capi.cpp).
+bool async_mode();
+bool strict_mode();
+unsigned int *status_buffer(hipError_t *err);
+int begin_call(hipStream_t s, unsigned int **st);
+int read_status(hipStream_t s, unsigned int *st, bool clear);
+int end_call(hipStream_t s, unsigned int *st, hipError_t launch);
+int check_grid(const spic_grid *g);
+int check_species(const spic_species *sp, bool need_q, bool need_move);
+int check_field(const spic_field *f);
+int check_moments(const spic_moments *m);
+int check_push(const spic_push *p);
+
 // Error reporting shared by every entry point: sets the thread-local
 // message returned by spic_last_error() and returns `code`.
 int fail(int code, const char *fmt, ...);

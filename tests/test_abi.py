@@ -43,6 +43,8 @@ STRUCTS = {
     "spic_push": _capi.spic_push,
     "spic_gem_params": _capi.spic_gem_params,
     "spic_gem_species": _capi.spic_gem_species,
+    "spic_engine_opts": _capi.spic_engine_opts,
+    "spic_engine_stats": _capi.spic_engine_stats,
 }
 
 

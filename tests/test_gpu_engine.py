@@ -1,0 +1,172 @@
+"""GPU parity of the tile-resident engine (spic_engine_* through the C ABI):
+one launch per particle phase over tile-sorted particles, LDS-staged fields,
+LDS moment accumulation.
+
+Tolerances: in strict arithmetic the particle state after any number of
+phases is bit-identical to the oracle (and so to the reference), whatever
+the tile shape and sort schedule, because sorting only reorders particles
+and the arithmetic per particle is the reference's; node moments then
+differ only in summation order (LDS partials + FP64 global adds): 1e-13 of
+each array's max magnitude in FP64, 1e-5 in FP32 (the FP32-form bound).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sputnipic_amd as S
+from helpers import (CASES, PRECS, assert_bitwise, assert_close, load_case,
+                     oracle_from_fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _modes():
+    S.set_async(False)
+    S.set_strict_fp(False)
+    yield
+    S.set_strict_fp(False)
+
+
+_CACHE = {}
+
+
+def oracle_small(prec):
+    if prec not in _CACHE:
+        _CACHE[prec] = O.State(O.GemConfig(), prec)
+    fresh = O.State(_CACHE[prec].cfg, prec, build_ic=False)
+    fresh.copy_from(_CACHE[prec])
+    return fresh
+
+
+def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0):
+    c = st.cfg
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
+    dt = np.float64 if prec == "f64" else np.float32
+    field = S.EMField(grd, {n: st.fields[n].astype(dt) for n in O.FIELD_NAMES})
+    parts = [S.Particles(p, c.qom[s], prec, c.NiterMover) for s, p in enumerate(st.parts)]
+    ids = [S.InterpDensSpecies(grd) for _ in range(c.ns)]
+    param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
+                         PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
+    eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every)
+    for s, p in enumerate(parts):
+        eng.load(s, p)
+    for _ in range(phases):
+        S.setZeroSpeciesDensities(ids, grd, c.ns)
+        eng.phase(field, ids, param)
+    for s, p in enumerate(parts):
+        eng.store(s, p)
+    out_p = [p.download() for p in parts]
+    out_m = [m.download() for m in ids]
+    stats = eng.stats()
+    eng.close()
+    return out_p, out_m, stats
+
+
+def oracle_phases(st: O.State, phases: int):
+    for _ in range(phases):
+        st.cycle()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_strict_matches_oracle_bitwise(prec):
+    st = oracle_small(prec)
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, prec, phases=3)
+    oracle_phases(st, 3)
+    assert stats["phases"] == 3
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
+@pytest.mark.parametrize("tile,sort_every", [((8, 4, 4), 1), ((2, 2, 2), 2), ((4, 4, 4), 1)])
+def test_engine_tile_shapes_and_sorting_do_not_change_trajectories(tile, sort_every):
+    st = oracle_small("f64")
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, "f64", phases=3, tile=tile, sort_every=sort_every)
+    oracle_phases(st, 3)
+    assert stats["tile_x"] == tile[0]
+    assert stats["sorts"] >= st.cfg.ns * (1 + (2 // sort_every))
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", CASES)
+def test_engine_fixture_cycles(name, prec):
+    """Reference fixtures: two phases, then compare with cycle 2 of the
+    reference run (mover_PC + interpP2G twice from the same IC)."""
+    data = load_case(name, prec)
+    st = oracle_from_fixture(name, prec, data)
+    S.set_strict_fp(True)
+    got_p, got_m, _ = run_engine(st, prec, phases=2, tile=(2, 2, 2))
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], data[f"c2_s{s}_{k}"], f"c2 s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], data[f"c2_s{s}_{n}"], 1e-13 if prec == "f64" else 1e-5,
+                         f"c2 s{s} {n}")
+
+
+def test_engine_stale_tiles_take_the_global_path():
+    """gem_hot particles (E set to zero so speeds stay below one cell per
+    step) cross cells every phase; without re-sorting they leave their
+    tile's halo and must still be pushed and deposited exactly."""
+    data = load_case("gem_hot", "f64")
+    st = oracle_from_fixture("gem_hot", "f64", data)
+    for n in ("Ex", "Ey", "Ez"):
+        st.fields[n][:] = 0.0
+    x0 = [p["x"].copy() for p in st.parts]
+    S.set_strict_fp(True)
+    got_p, got_m, _ = run_engine(st, "f64", phases=8, tile=(2, 2, 2), sort_every=0)
+    oracle_phases(st, 8)
+    dx = st.cfg.Lx / st.cfg.nxc
+    moved = sum(int(np.sum(np.abs(p["x"] - x) > 3 * dx)) for p, x in zip(st.parts, x0))
+    assert moved > 100, moved  # many particles are outside their tile's halo
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_fast_against_f64_oracle(prec):
+    st = oracle_small("f64")
+    got_p, got_m, _ = run_engine(st, prec, phases=2, sort_every=1)
+    oracle_phases(st, 2)
+    tol = 1e-12 if prec == "f64" else 1e-5
+    for s in range(st.cfg.ns):
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], tol, f"s{s} {n}")
+        for k in ("u", "v", "w"):
+            assert_close(got_p[s][k], st.parts[s][k], tol, f"s{s} {k}")
+        for k in ("x", "y", "z"):
+            assert np.max(np.abs(got_p[s][k] - st.parts[s][k])) < (1e-11 if prec == "f64"
+                                                                   else 1e-5), k
+
+
+def test_engine_conserves_charge():
+    st = oracle_small("f64")
+    got_p, got_m, _ = run_engine(st, "f64", phases=1)
+    iv = O.State(st.cfg, "f64", build_ic=False).grid.invVOL
+    for s in range(st.cfg.ns):
+        q = got_p[s]["q"]
+        assert abs(np.sum(got_m[s]["rhon"]) / iv - np.sum(q)) <= 1e-12 * np.sum(np.abs(q))
+
+
+def test_engine_reports_out_of_domain():
+    st = oracle_small("f64")
+    st.parts[0]["x"][7] = -5.0
+    with pytest.raises(S.SpicError) as ei:
+        run_engine(st, "f64", phases=1)
+    assert ei.value.code == S.SPIC_EDOMAIN
