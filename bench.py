@@ -197,6 +197,28 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
     return out
 
 
+class _StdoutToStderr:
+    """Route the process's C-level stdout to stderr, so that text the CPU
+    baseline's compiled code prints (the reference's initGEM writes a banner
+    with std::cout) never mixes into the one JSON line on stdout."""
+
+    def __enter__(self):
+        import ctypes
+        self.libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def cpu_baseline(cycles: int) -> dict:
     """The reference's own CPU mover_PC + interpP2G (its source compiled at
     the optimisation level of its Makefile, oracle/_ref/libref_f64_O3.so),
@@ -247,7 +269,19 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--tile", default="4x4x4")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES once device to device (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import sputnipic_amd as S
+        from sputnipic_amd._capi import check, lib
+        n = args.copy_probe // 4
+        a = S.DeviceArray(n, "float32", fill_zero=True)
+        b = S.DeviceArray(n, "float32")
+        check(lib().spic_memcpy_d2d(b.ptr, a.ptr, n * 4), "copy probe")
+        print(f"copy probe: {n * 4} bytes read and {n * 4} bytes written", file=sys.stderr)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus}: this build measures one GPU only; the path "
               "would shard by particles or by x-slabs (DESIGN.md), which is not built "
@@ -269,7 +303,8 @@ def main() -> int:
     f = forms[primary]
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_cycles)
+        with _StdoutToStderr():
+            cpu = cpu_baseline(args.cpu_cycles)
     line = {
         "metric": METRIC,
         "value": f["value"],

@@ -1,0 +1,358 @@
+// spic_dropin.cpp -- the reference's C++ entry points (mover_PC, interpP2G,
+// mover_interp, particle_sort) over the C ABI of libspic.so, for host
+// structs laid out as the reference's (include/sputnipic/sputnipic.h).
+// Built twice by the Makefile: libspic_dropin_f32.so (the reference's
+// default float precision) and libspic_dropin_f64.so (-DSPIC_REF_FP64).
+//
+// Device mirrors are created lazily on first use and keyed by the host
+// array they mirror (grid: XN_flat, fields: Ex_flat, species: x, moments:
+// rhon_flat); the caller owns and never loses its host memory.  Calls are
+// thread-safe: the registry is mutex-protected and every calling thread
+// uses its own stream, as the reference calls interpP2G from an OpenMP team
+// with one thread per species (src/sputniPIC_CPU.cpp:250-256).
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "spic.h"
+#include "sputnipic/sputnipic.h"
+
+namespace {
+
+constexpr int kPrec = sizeof(FPpart) == 8 ? SPIC_F64 : SPIC_F32;
+static_assert(sizeof(FPpart) == sizeof(FPfield) && sizeof(FPpart) == sizeof(FPinterp),
+              "the drop-in is built for one precision throughout");
+
+thread_local int t_status = SPIC_OK;
+thread_local void *t_stream = nullptr;
+std::mutex g_mu;
+int g_resident = 0;
+
+struct GridMirror {
+  spic_grid g;
+  void *xn = nullptr, *yn = nullptr, *zn = nullptr;
+};
+struct FieldMirror {
+  void *d[6] = {};
+  size_t nn = 0;
+};
+struct PartMirror {
+  void *d[7] = {};
+  long npmax = 0;
+  bool device_current = false;  // device holds the newest particle state
+  bool host_current = true;      // host arrays hold the newest state
+};
+struct MomMirror {
+  double *d[10] = {};
+  size_t nn = 0;
+};
+
+std::map<const void *, GridMirror> g_grids;
+std::map<const void *, FieldMirror> g_fields;
+std::map<const void *, PartMirror> g_parts;
+std::map<const void *, MomMirror> g_moms;
+
+int report(const char *fn, int rc) {
+  t_status = rc;
+  if (rc != SPIC_OK)
+    std::fprintf(stderr, "sputnipic drop-in: %s: %s\n", fn, spic_last_error());
+  return rc;
+}
+
+void *stream() {
+  if (!t_stream && spic_stream_create(&t_stream) != SPIC_OK) t_stream = nullptr;
+  return t_stream;
+}
+
+// The reference's node arrays depend on one index each
+// (src/Grid.cpp:53-61); the device uses the 1-D tables.
+int grid_mirror(const grid *grd, GridMirror **out) {
+  auto it = g_grids.find(grd->XN_flat);
+  if (it != g_grids.end()) {
+    *out = &it->second;
+    return SPIC_OK;
+  }
+  const int nxn = grd->nxn, nyn = grd->nyn, nzn = grd->nzn;
+  std::vector<FPfield> xn(nxn), yn(nyn), zn(nzn);
+  for (int i = 0; i < nxn; ++i) xn[i] = grd->XN_flat[(long)i * nyn * nzn];
+  for (int j = 0; j < nyn; ++j) yn[j] = grd->YN_flat[(long)j * nzn];
+  for (int k = 0; k < nzn; ++k) zn[k] = grd->ZN_flat[k];
+  for (int i = 0; i < nxn; ++i)
+    for (int j = 0; j < nyn; ++j)
+      for (int k = 0; k < nzn; ++k) {
+        const long n = ((long)i * nyn + j) * nzn + k;
+        if (grd->XN_flat[n] != xn[i] || grd->YN_flat[n] != yn[j] ||
+            grd->ZN_flat[n] != zn[k])
+          return SPIC_EINVAL;  // not the separable lattice setGrid builds
+      }
+  GridMirror m;
+  int rc;
+  if ((rc = spic_malloc(&m.xn, sizeof(FPfield) * nxn)) ||
+      (rc = spic_malloc(&m.yn, sizeof(FPfield) * nyn)) ||
+      (rc = spic_malloc(&m.zn, sizeof(FPfield) * nzn)) ||
+      (rc = spic_memcpy_h2d(m.xn, xn.data(), sizeof(FPfield) * nxn)) ||
+      (rc = spic_memcpy_h2d(m.yn, yn.data(), sizeof(FPfield) * nyn)) ||
+      (rc = spic_memcpy_h2d(m.zn, zn.data(), sizeof(FPfield) * nzn)))
+    return rc;
+  spic_grid &g = m.g;
+  g.nxc = grd->nxc; g.nyc = grd->nyc; g.nzc = grd->nzc;
+  g.nxn = nxn; g.nyn = nyn; g.nzn = nzn;
+  g.dx = grd->dx; g.dy = grd->dy; g.dz = grd->dz;
+  g.invdx = grd->invdx; g.invdy = grd->invdy; g.invdz = grd->invdz;
+  g.invVOL = grd->invVOL;
+  g.xStart = grd->xStart; g.yStart = grd->yStart; g.zStart = grd->zStart;
+  g.Lx = grd->Lx; g.Ly = grd->Ly; g.Lz = grd->Lz;
+  g.prec = kPrec;
+  g.xn = m.xn; g.yn = m.yn; g.zn = m.zn;
+  auto res = g_grids.emplace(grd->XN_flat, m);
+  *out = &res.first->second;
+  return SPIC_OK;
+}
+
+// Fields are uploaded on every call: the reference's field solver rewrites
+// them on the host between cycles (src/sputniPIC_CPU.cpp:288-289).
+int field_upload(const EMfield *f, size_t nn, spic_field *out) {
+  FieldMirror &m = g_fields[f->Ex_flat];
+  int rc;
+  if (!m.d[0]) {
+    for (int a = 0; a < 6; ++a)
+      if ((rc = spic_malloc(&m.d[a], sizeof(FPfield) * nn))) return rc;
+    m.nn = nn;
+  }
+  const FPfield *src[6] = {f->Ex_flat, f->Ey_flat, f->Ez_flat,
+                           f->Bxn_flat, f->Byn_flat, f->Bzn_flat};
+  for (int a = 0; a < 6; ++a)
+    if ((rc = spic_memcpy_h2d(m.d[a], src[a], sizeof(FPfield) * nn))) return rc;
+  out->Ex = m.d[0]; out->Ey = m.d[1]; out->Ez = m.d[2];
+  out->Bxn = m.d[3]; out->Byn = m.d[4]; out->Bzn = m.d[5];
+  return SPIC_OK;
+}
+
+FPpart *const *host_arrays(const particles *p, FPpart *arr[7]) {
+  arr[0] = p->x; arr[1] = p->y; arr[2] = p->z;
+  arr[3] = p->u; arr[4] = p->v; arr[5] = p->w;
+  arr[6] = (FPpart *)p->q;
+  return arr;
+}
+
+// Device copy of a species, current with respect to the host arrays.
+int part_mirror(const particles *p, PartMirror **out, spic_species *sp) {
+  PartMirror &m = g_parts[p->x];
+  int rc;
+  if (!m.d[0] || m.npmax != p->npmax) {
+    for (int a = 0; a < 7; ++a)
+      if (m.d[a]) spic_free(m.d[a]);
+    for (int a = 0; a < 7; ++a)
+      if ((rc = spic_malloc(&m.d[a], sizeof(FPpart) * (size_t)p->npmax))) return rc;
+    m.npmax = p->npmax;
+    m.device_current = false;
+    m.host_current = true;
+  }
+  // coherent mode: the host arrays are the reference copy on every call
+  if (!m.device_current || !g_resident) {
+    FPpart *arr[7];
+    host_arrays(p, arr);
+    for (int a = 0; a < 7; ++a)
+      if ((rc = spic_memcpy_h2d(m.d[a], arr[a], sizeof(FPpart) * (size_t)p->npmax)))
+        return rc;
+    m.device_current = true;
+  }
+  sp->x = m.d[0]; sp->y = m.d[1]; sp->z = m.d[2];
+  sp->u = m.d[3]; sp->v = m.d[4]; sp->w = m.d[5]; sp->q = m.d[6];
+  sp->nop = p->nop;
+  sp->npmax = p->npmax;
+  sp->qom = p->qom;
+  sp->NiterMover = p->NiterMover;
+  sp->n_sub_cycles = p->n_sub_cycles;
+  *out = &m;
+  return SPIC_OK;
+}
+
+// After a call that changed particles: coherent mode copies them back.
+int part_written(const particles *p, PartMirror *m, bool q_too) {
+  m->host_current = false;
+  if (g_resident) return SPIC_OK;
+  FPpart *arr[7];
+  host_arrays(p, arr);
+  const int na = q_too ? 7 : 6;
+  for (int a = 0; a < na; ++a) {
+    int rc = spic_memcpy_d2h(arr[a], m->d[a], sizeof(FPpart) * (size_t)p->npmax);
+    if (rc) return rc;
+  }
+  m->host_current = true;
+  return SPIC_OK;
+}
+
+int mom_mirror(const interpDensSpecies *ids, size_t nn, MomMirror **out,
+               spic_moments *mom) {
+  MomMirror &m = g_moms[ids->rhon_flat];
+  int rc;
+  if (!m.d[0]) {
+    for (int a = 0; a < 10; ++a)
+      if ((rc = spic_malloc((void **)&m.d[a], sizeof(double) * nn))) return rc;
+    m.nn = nn;
+  }
+  mom->rhon = m.d[0]; mom->Jx = m.d[1]; mom->Jy = m.d[2]; mom->Jz = m.d[3];
+  mom->pxx = m.d[4]; mom->pxy = m.d[5]; mom->pxz = m.d[6];
+  mom->pyy = m.d[7]; mom->pyz = m.d[8]; mom->pzz = m.d[9];
+  *out = &m;
+  return SPIC_OK;
+}
+
+// interpP2G adds into the caller's arrays (src/Mover.cpp:469-570): the
+// device sum of this call is added to each host node value.
+int mom_add_to_host(const interpDensSpecies *ids, MomMirror *m) {
+  FPinterp *dst[10] = {ids->rhon_flat, ids->Jx_flat,  ids->Jy_flat,  ids->Jz_flat,
+                       ids->pxx_flat,  ids->pxy_flat, ids->pxz_flat, ids->pyy_flat,
+                       ids->pyz_flat,  ids->pzz_flat};
+  std::vector<double> buf(m->nn);
+  for (int a = 0; a < 10; ++a) {
+    int rc = spic_memcpy_d2h(buf.data(), m->d[a], sizeof(double) * m->nn);
+    if (rc) return rc;
+    for (size_t n = 0; n < m->nn; ++n) dst[a][n] = (FPinterp)(dst[a][n] + buf[n]);
+  }
+  return SPIC_OK;
+}
+
+spic_push push_of(const parameters *param) {
+  spic_push p;
+  p.dt = param->dt;
+  p.c = param->c;
+  p.PERIODICX = param->PERIODICX;
+  p.PERIODICY = param->PERIODICY;
+  p.PERIODICZ = param->PERIODICZ;
+  return p;
+}
+
+size_t nodes(const grid *grd) { return (size_t)grd->nxn * grd->nyn * grd->nzn; }
+
+}  // namespace
+
+int mover_PC(struct particles *part, struct EMfield *field, struct grid *grd,
+             struct parameters *param) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  GridMirror *gm;
+  PartMirror *pm;
+  spic_field f;
+  spic_species sp;
+  int rc = grid_mirror(grd, &gm);
+  if (rc == SPIC_OK) rc = field_upload(field, nodes(grd), &f);
+  if (rc == SPIC_OK) rc = part_mirror(part, &pm, &sp);
+  if (rc == SPIC_OK) {
+    const spic_push p = push_of(param);
+    rc = spic_mover_pc(&gm->g, &f, &sp, &p, stream());
+  }
+  if (rc == SPIC_OK) rc = part_written(part, pm, false);
+  return report("mover_PC", rc);
+}
+
+void interpP2G(struct particles *part, struct interpDensSpecies *ids,
+               struct grid *grd) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  GridMirror *gm;
+  PartMirror *pm;
+  MomMirror *mm;
+  spic_species sp;
+  spic_moments mom;
+  int rc = grid_mirror(grd, &gm);
+  if (rc == SPIC_OK) rc = part_mirror(part, &pm, &sp);
+  if (rc == SPIC_OK) rc = mom_mirror(ids, nodes(grd), &mm, &mom);
+  if (rc == SPIC_OK) rc = spic_zero_moments(&gm->g, &mom, stream());
+  if (rc == SPIC_OK) rc = spic_interp_p2g(&gm->g, &sp, &mom, stream());
+  if (rc == SPIC_OK) rc = mom_add_to_host(ids, mm);
+  report("interpP2G", rc);
+}
+
+int mover_interp(struct particles *part, struct EMfield *field,
+                 struct interpDensSpecies *ids, struct grid *grd,
+                 struct parameters *param) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  GridMirror *gm;
+  PartMirror *pm;
+  MomMirror *mm;
+  spic_field f;
+  spic_species sp;
+  spic_moments mom;
+  int rc = grid_mirror(grd, &gm);
+  if (rc == SPIC_OK) rc = field_upload(field, nodes(grd), &f);
+  if (rc == SPIC_OK) rc = part_mirror(part, &pm, &sp);
+  if (rc == SPIC_OK) rc = mom_mirror(ids, nodes(grd), &mm, &mom);
+  if (rc == SPIC_OK) rc = spic_zero_moments(&gm->g, &mom, stream());
+  if (rc == SPIC_OK) {
+    const spic_push p = push_of(param);
+    rc = spic_mover_interp(&gm->g, &f, &sp, &mom, &p, stream());
+  }
+  if (rc == SPIC_OK) rc = part_written(part, pm, false);
+  if (rc == SPIC_OK) rc = mom_add_to_host(ids, mm);
+  return report("mover_interp", rc);
+}
+
+void particle_sort(struct parameters *params, struct particles *particles,
+                   struct grid *grd) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  GridMirror *gm;
+  PartMirror *pm;
+  spic_species sp;
+  int rc = grid_mirror(grd, &gm);
+  if (rc == SPIC_OK) rc = part_mirror(particles, &pm, &sp);
+  if (rc == SPIC_OK) rc = spic_particle_sort(&gm->g, params->sort_cps, &sp, stream());
+  if (rc == SPIC_OK) rc = part_written(particles, pm, true);
+  report("particle_sort", rc);
+}
+
+extern "C" {
+
+int spic_dropin_set_resident(int resident) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_resident = resident ? 1 : 0;
+  return SPIC_OK;
+}
+
+int spic_dropin_sync_to_host(const struct particles *part) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_parts.find(part->x);
+  if (it == g_parts.end() || it->second.host_current) return SPIC_OK;
+  FPpart *arr[7];
+  host_arrays(part, arr);
+  for (int a = 0; a < 7; ++a) {
+    int rc = spic_memcpy_d2h(arr[a], it->second.d[a],
+                             sizeof(FPpart) * (size_t)part->npmax);
+    if (rc) return report("spic_dropin_sync_to_host", rc);
+  }
+  it->second.host_current = true;
+  return SPIC_OK;
+}
+
+int spic_dropin_host_modified(const struct particles *part) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_parts.find(part->x);
+  if (it != g_parts.end()) it->second.device_current = false;
+  return SPIC_OK;
+}
+
+int spic_dropin_last_status(void) { return t_status; }
+
+int spic_dropin_release_all(void) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  spic_synchronize();
+  for (auto &kv : g_grids) {
+    spic_free(kv.second.xn);
+    spic_free(kv.second.yn);
+    spic_free(kv.second.zn);
+  }
+  for (auto &kv : g_fields)
+    for (void *p : kv.second.d) spic_free(p);
+  for (auto &kv : g_parts)
+    for (void *p : kv.second.d) spic_free(p);
+  for (auto &kv : g_moms)
+    for (double *p : kv.second.d) spic_free(p);
+  g_grids.clear();
+  g_fields.clear();
+  g_parts.clear();
+  g_moms.clear();
+  return SPIC_OK;
+}
+
+}  // extern "C"
