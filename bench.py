@@ -76,12 +76,13 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         for s, p in enumerate(parts):
             eng.load(s, p, stream)
         stream.synchronize()
-        log(f"{prec}: loaded + tile-sorted in {time.perf_counter() - t0:.2f} s")
+        log(f"{prec}: loaded + binned in {time.perf_counter() - t0:.2f} s")
         for p in parts:
             for d in p.d.values():
                 d.free()
         parts = None
-        kernel = f"k_tiled_phase<{prec}, {tile[0]}x{tile[1]}x{tile[2]}> (all species, one launch)"
+        kernel = (f"k_binned_phase<{prec}, {tile[0]}x{tile[1]}x{tile[2]}> (all species, one "
+                  f"launch; timed with its overflow, reduce and count-reset launches)")
     else:
         s_c = [p.c_struct() for p in parts]
         kernel = "k_mover_deposit (plain fused, one launch per species)"
@@ -188,7 +189,12 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         "charge_rel_err": q_err,
     }
     if eng is not None:
-        out["engine_device_bytes"] = eng.stats()["device_bytes"]
+        st = eng.stats()
+        out["engine_device_bytes"] = st["device_bytes"]
+        out["engine_stats"] = {k: st[k] for k in ("phases", "sorts", "lds_gathers",
+                                                  "global_gathers", "lds_deposits",
+                                                  "global_deposits", "overflow")}
+        log(f"{prec}: engine stats {out['engine_stats']}")
         eng.close()
     else:
         for p in parts:
@@ -267,7 +273,7 @@ def main() -> int:
     ap.add_argument("--cpu-cycles", type=int, default=10)
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
-    ap.add_argument("--sort-every", type=int, default=10)
+    ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once device to device (calibrates the "

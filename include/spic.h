@@ -145,26 +145,47 @@ int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
 /* setZeroSpeciesDensities, node-moment part: zero the ten arrays. */
 int spic_zero_moments(const spic_grid *grd, spic_moments *mom, void *stream);
 
+/* applyBCids (src/BC.cpp:362-619) on `ns` consecutive moment sets (1..8):
+ * fold the ghost-node moments in place, the periodic axes (push->PERIODICX,
+ * Y, Z) first in the order x, y, z (a[1] += a[n-2]; a[n-2] = a[1]), then the
+ * other axes in the same order (a[1] *= 2; a[n-2] *= 2), each over the
+ * interior of the two other axes.  Bit-identical to the FP64 reference. */
+int spic_apply_bc_ids(const spic_grid *grd, const spic_moments *mom, int ns,
+                      const spic_push *push, void *stream);
+
+/* sumOverSpecies (src/InterpDensNet.cpp:120-144): net += ids[0] + ... +
+ * ids[ns-1] (1..8 sets), added in species order per element, ten node
+ * moments.  Bit-identical to the FP64 reference. */
+int spic_sum_species(const spic_grid *grd, const spic_moments *net,
+                     const spic_moments *ids, int ns, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Tile-resident engine: the whole particle phase of a cycle -- mover_PC   */
 /* then interpP2G for every species, the loop of                          */
-/* src/sputniPIC_CPU.cpp:232-256 -- in one launch over tile-sorted,        */
-/* HBM-resident particles.  The engine owns its particle storage (two      */
-/* buffer sets, for the sort) and a particle id per slot, so particles can */
-/* be copied out in the order they were loaded.  Internally particles are  */
-/* re-sorted by tile every `sort_every` phases; the sort changes only the  */
-/* summation order of the moments, never a particle's trajectory.          */
+/* src/sputniPIC_CPU.cpp:232-256 -- in one launch over HBM-resident        */
+/* particles binned by tile.  The engine owns its particle storage: per    */
+/* species two buffers with one segment per tile (its particles plus free  */
+/* slots) and an overflow region, and a particle id per slot, so particles */
+/* can be copied out in the order they were loaded.  Each phase writes     */
+/* every particle into its destination tile's segment of the other buffer, */
+/* so the binning stays current without a separate sort.  A sort (on load, */
+/* every `sort_every` phases, or on request) rebuilds the segments.        */
+/* Neither changes a particle's trajectory, only the summation order of    */
+/* the moments.                                                            */
 /* ---------------------------------------------------------------------- */
 typedef struct spic_engine spic_engine;
 
 typedef struct spic_engine_opts {
-  int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s), 8x4x4
-                                 or 2x2x2 */
-  int sort_every;             /* re-sort every N phases; <= 0: only on load */
-  int lds_partials;           /* precision of the per-tile LDS moment partials
+  int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s) or 8x4x4 */
+  int sort_every;             /* rebuild the segments every N phases; <= 0:
+                                 only on load and spic_engine_sort */
+  int lds_partials;           /* precision of the per-tile LDS moment block
                                  of an FP32 engine: 0 = default (FP64),
                                  1 = FP32, 2 = FP64.  An FP64 engine always
                                  uses FP64. */
+  int slack_pct;              /* free slots per segment after a sort, in
+                                 percent of the mean particles per tile
+                                 (at least 16 slots); 0 = default (12) */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {
@@ -175,10 +196,11 @@ typedef struct spic_engine_stats {
   size_t device_bytes;  /* device memory held by the engine */
   /* cumulative over all phases: predictor-corrector stencil gathers and
    * particle deposits served from the LDS-staged tile, and those that fell
-   * outside it (particle more than one cell out of its tile since the last
-   * sort) and went to global memory */
+   * outside it (a particle more than one cell out of its tile, or in an
+   * overflow region) and went to global memory */
   int64_t lds_gathers, global_gathers;
   int64_t lds_deposits, global_deposits;
+  int64_t overflow;     /* particles now in overflow regions (all species) */
 } spic_engine_stats;
 
 /* Create an engine for `ns` species (1..8) on `grd` (its node tables are
@@ -187,7 +209,7 @@ int spic_engine_create(spic_engine **eng, const spic_grid *grd, int ns,
                        const spic_engine_opts *opts);
 int spic_engine_destroy(spic_engine *eng);
 /* Copy species `is` in (device arrays, caller order; nop particles, qom,
- * NiterMover, n_sub_cycles) and sort it into tile order. */
+ * NiterMover, n_sub_cycles) and sort it into tile segments. */
 int spic_engine_load(spic_engine *eng, int is, const spic_species *sp,
                      void *stream);
 /* Copy species `is` out into sp's arrays in the order it was loaded
@@ -200,7 +222,7 @@ int spic_engine_store(spic_engine *eng, int is, spic_species *sp,
 int spic_engine_phase(spic_engine *eng, const spic_field *field,
                       const spic_moments *mom, const spic_push *push,
                       void *stream);
-/* Re-sort every species now. */
+/* Rebuild every species' tile segments now. */
 int spic_engine_sort(spic_engine *eng, void *stream);
 int spic_engine_get_stats(const spic_engine *eng, spic_engine_stats *st);
 

@@ -339,6 +339,20 @@ class State:
             C.byref(self.grid), C.byref(ms), int(c.PERIODICX), int(c.PERIODICY),
             int(c.PERIODICZ))
 
+    def sum_species(self, net: Optional[Dict[str, np.ndarray]] = None) -> Dict[str, np.ndarray]:
+        """sumOverSpecies: net += every species' moments, in species order;
+        net starts at zero unless given."""
+        dt = self.moments[0]["rhon"].dtype
+        if net is None:
+            net = {n: np.zeros(self.nn, dt) for n in MOMENT_NAMES}
+        net = {n: np.ascontiguousarray(net[n], dt).copy() for n in MOMENT_NAMES}
+        ns_ = self.cfg.ns
+        ids = (OMoments * ns_)(*[self.mom_struct(i) for i in range(ns_)])
+        out = OMoments(*[_ptr(net[n]) for n in MOMENT_NAMES])
+        getattr(lib(), f"oracle_sum_species_{self.prec}")(C.byref(self.grid), C.byref(out),
+                                                           ids, ns_)
+        return net
+
     def cycle(self) -> None:
         """One cycle of the hot path as src/sputniPIC_CPU.cpp:232-256 runs
         it: mover_PC for every species, zero the moments, interpP2G for every
@@ -466,6 +480,7 @@ class RefState:
             for f in ("ref_interp_p2g", "ref_particle_sort", "ref_apply_bc_ids"):
                 getattr(L, f).argtypes = [_P, C.c_int]
             L.ref_zero_species_moments.argtypes = [_P]
+            L.ref_sum_species.argtypes = [_P, C.POINTER(C.c_double)]
             L.ref_time_cycles.argtypes = [_P, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
             L.ref_time_cycles.restype = C.c_double
@@ -537,3 +552,10 @@ class RefState:
 
     def apply_bc_ids(self, is_):
         self.L.ref_apply_bc_ids(self.h, is_)
+
+    def sum_species(self) -> Dict[str, np.ndarray]:
+        """sumOverSpecies into a zeroed net; the ten node moments (float64)."""
+        nn = self.nn
+        out = np.zeros(10 * nn, np.float64)
+        self.L.ref_sum_species(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return {n: out[i * nn:(i + 1) * nn].copy() for i, n in enumerate(MOMENT_NAMES)}

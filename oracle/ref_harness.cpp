@@ -10,7 +10,8 @@
 //   (src/InterpDensSpecies.cpp:4), particle_allocate (src/Particles.cpp:5),
 //   initGEM (src/IC.cpp:7), mover_PC / interpP2G / mover_interp
 //   (src/Mover.cpp), particle_sort (src/Particles.cpp:72), applyBCids
-//   (src/BC.cpp:362), setZeroSpeciesDensities (src/InterpDensNet.cpp:40).
+//   (src/BC.cpp:362), setZeroSpeciesDensities (src/InterpDensNet.cpp:40),
+//   setZeroNetDensities / sumOverSpecies (src/InterpDensNet.cpp:78-144).
 // It fills struct parameters directly, with the defaults of
 // src/RW_IO.cpp:37-295 for keys it does not set, because readInputFile
 // lives in RW_IO.cpp, which needs HDF5 (absent here).
@@ -255,6 +256,23 @@ void ref_particle_sort(void *h, int is) {
 void ref_apply_bc_ids(void *h, int is) {
   ref_state *s = (ref_state *)h;
   applyBCids(&s->ids[is], &s->grd, &s->param);
+}
+
+// sumOverSpecies (src/InterpDensNet.cpp:120-144) into a zeroed net
+// (setZeroNetDensities, src/InterpDensNet.cpp:78-110); out receives the ten
+// node moments, each nn long, in the order rhon Jx Jy Jz pxx .. pzz.
+void ref_sum_species(void *h, double *out) {
+  ref_state *s = (ref_state *)h;
+  interpDensNet idn;
+  interp_dens_net_allocate(&s->grd, &idn);
+  setZeroNetDensities(&idn, &s->grd);
+  sumOverSpecies(&idn, s->ids, &s->grd, s->param.ns);
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  FPinterp *a[10] = {idn.rhon_flat, idn.Jx_flat,  idn.Jy_flat,  idn.Jz_flat,
+                     idn.pxx_flat,  idn.pxy_flat, idn.pxz_flat, idn.pyy_flat,
+                     idn.pyz_flat,  idn.pzz_flat};
+  for (int m = 0; m < 10; ++m) to_double(a[m], out + m * nn, nn);
+  interp_dens_net_deallocate(&s->grd, &idn);
 }
 
 }  // extern "C"

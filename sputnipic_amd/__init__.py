@@ -8,13 +8,15 @@ reference's interface over that ABI.  There is no CPU fallback.
 """
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
 from .core import (DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams,  # noqa: F401
-                   GemSpecies, Grid, InterpDensSpecies, Parameters, Particles, Stream,
-                   FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, device_count, gem_fields,
-                   gem_particles, harris_species, interpP2G, mover_interp, mover_PC,
-                   particle_sort, set_async, set_strict_fp, setZeroSpeciesDensities)
+                   GemSpecies, Grid, InterpDensNet, InterpDensSpecies, Parameters,
+                   Particles, Stream, FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, applyBCids,
+                   applyBCids_all, device_count, gem_fields, gem_particles, harris_species,
+                   interpP2G, mover_interp, mover_PC, particle_sort, set_async,
+                   set_strict_fp, setZeroSpeciesDensities, sumOverSpecies)
 
 __all__ = [
     "SpicError", "DeviceArray", "EMField", "Grid", "InterpDensSpecies", "Parameters",
     "Particles", "mover_PC", "interpP2G", "mover_interp", "particle_sort",
-    "setZeroSpeciesDensities", "set_strict_fp", "set_async", "device_count",
+    "setZeroSpeciesDensities", "applyBCids", "sumOverSpecies", "InterpDensNet",
+    "set_strict_fp", "set_async", "device_count",
 ]

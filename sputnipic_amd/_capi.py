@@ -82,7 +82,8 @@ class spic_gem_species(C.Structure):
 
 class spic_engine_opts(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
-                ("sort_every", C.c_int), ("lds_partials", C.c_int)]
+                ("sort_every", C.c_int), ("lds_partials", C.c_int),
+                ("slack_pct", C.c_int)]
 
 
 class spic_engine_stats(C.Structure):
@@ -90,7 +91,8 @@ class spic_engine_stats(C.Structure):
                 ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
                 ("device_bytes", C.c_size_t),
                 ("lds_gathers", C.c_int64), ("global_gathers", C.c_int64),
-                ("lds_deposits", C.c_int64), ("global_deposits", C.c_int64)]
+                ("lds_deposits", C.c_int64), ("global_deposits", C.c_int64),
+                ("overflow", C.c_int64)]
 
 
 # name -> (restype, argtypes); every entry point of include/spic.h
@@ -105,6 +107,10 @@ SIGNATURES = {
     "spic_particle_sort": (C.c_int, [C.POINTER(spic_grid), C.c_int64,
                                      C.POINTER(spic_species), _P]),
     "spic_zero_moments": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments), _P]),
+    "spic_apply_bc_ids": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments), C.c_int,
+                                    C.POINTER(spic_push), _P]),
+    "spic_sum_species": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments),
+                                   C.POINTER(spic_moments), C.c_int, _P]),
     "spic_engine_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_grid), C.c_int,
                                      C.POINTER(spic_engine_opts)]),
     "spic_engine_destroy": (C.c_int, [_P]),

@@ -203,9 +203,18 @@ class InterpDensSpecies:
     def download(self) -> Dict[str, np.ndarray]:
         return {n: self.d[n].download() for n in MOMENT_NAMES}
 
+    def upload(self, arrays: Dict[str, np.ndarray]) -> None:
+        for n in MOMENT_NAMES:
+            self.d[n].upload(np.ascontiguousarray(arrays[n], np.float64))
+
     def c_struct(self) -> spic_moments:
         D = C.POINTER(C.c_double)
         return spic_moments(*[C.cast(self.d[n].ptr, D) for n in MOMENT_NAMES])
+
+
+class InterpDensNet(InterpDensSpecies):
+    """The net node moments summed over species (include/InterpDensNet.h:9-40,
+    the ten node moments), FP64 on the device."""
 
 
 def _check_prec(*objs):
@@ -264,6 +273,31 @@ def setZeroSpeciesDensities(ids: Sequence[InterpDensSpecies], grd: Grid, ns: int
     for is_ in range(ns):
         m = ids[is_].c_struct()
         check(lib().spic_zero_moments(C.byref(g), C.byref(m), None), "zero moments")
+
+
+def applyBCids(ids: InterpDensSpecies, grd: Grid, param: Parameters) -> None:
+    """applyBCids (src/BC.cpp:362-619): fold one species' ghost-node
+    moments in place on the device."""
+    g, m, p = grd.c_struct(), ids.c_struct(), param.push()
+    check(lib().spic_apply_bc_ids(C.byref(g), C.byref(m), 1, C.byref(p), None),
+          "applyBCids")
+
+
+def applyBCids_all(ids: Sequence[InterpDensSpecies], grd: Grid, param: Parameters) -> None:
+    """applyBCids for every species at once (one launch per fold step)."""
+    g, p = grd.c_struct(), param.push()
+    moms = (spic_moments * len(ids))(*[m.c_struct() for m in ids])
+    check(lib().spic_apply_bc_ids(C.byref(g), moms, len(ids), C.byref(p), None),
+          "applyBCids")
+
+
+def sumOverSpecies(idn: InterpDensNet, ids: Sequence[InterpDensSpecies], grd: Grid,
+                   ns: int) -> None:
+    """sumOverSpecies (src/InterpDensNet.cpp:120-144): idn += ids[0..ns-1]."""
+    g, net = grd.c_struct(), idn.c_struct()
+    moms = (spic_moments * ns)(*[ids[i].c_struct() for i in range(ns)])
+    check(lib().spic_sum_species(C.byref(g), C.byref(net), moms, ns, None),
+          "sumOverSpecies")
 
 
 def device_count() -> int:
@@ -386,15 +420,16 @@ class Event:
 
 class Engine:
     """Tile-resident engine (spic_engine_*): every species' mover_PC +
-    interpP2G in one launch over HBM-resident, tile-sorted particles."""
+    interpP2G in one launch over HBM-resident particles binned by tile."""
 
     def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
-                 lds_partials: int = 0):
+                 lds_partials: int = 0, slack_pct: int = 0):
         from ._capi import spic_engine_opts
         self.grd = grd
         self.ns = ns
         p = C.c_void_p()
-        opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials)
+        opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials,
+                                slack_pct)
         g = grd.c_struct()
         check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
               "spic_engine_create")

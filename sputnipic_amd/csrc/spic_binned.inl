@@ -1,0 +1,687 @@
+// spic_binned.inl -- the binned tile-resident phase kernels: every species'
+// mover_PC followed by its interpP2G, with each particle written back into
+// the segment of its destination tile (binned storage, spic_binned.hpp).
+//
+// k_binned_phase: one workgroup of four threads per cell per tile of
+// TX x TY x TZ cells.  It
+//   1. stages the six node fields and node coordinates of the tile plus a
+//      one-cell halo into LDS;
+//   2. for each species, batch by batch (one particle per thread):
+//      - pushes the particle (src/Mover.cpp:33-157) with the eight-node
+//        stencil gathered from LDS and kept in registers while the cell is
+//        unchanged (from global memory if it lies outside the staged block);
+//      - writes it into its destination tile's segment of the output buffer
+//        (one counter atomic per wave for the particles staying in this
+//        tile, one per particle for the others; the overflow region when the
+//        segment is full);
+//      - deposits it (src/Mover.cpp:399-570): a particle in one of the tile's
+//        cells goes into that cell's LDS bucket; one in a halo cell adds its
+//        80 contributions to the LDS moment block; one outside the block adds
+//        them to global memory;
+//      - after a barrier the four threads of each cell take the cell's
+//        bucket, each thread two of the cell's eight nodes, and accumulate
+//        their ten moments in registers;
+//   3. adds the register sums into the LDS moment block and stores the block
+//      to the tile-partial scratch, summed per node by k_binned_reduce.
+// k_binned_overflow: the particles of the input overflow regions, one per
+// thread, with global-memory gathers and deposits, re-binned the same way.
+//
+// The arithmetic per particle is exactly the reference's (the operation
+// order of spic_plain.inl); with SPIC_STRICT it matches it bit for bit.
+// Only the order in which moment contributions are summed differs.
+#include "spic_binned.hpp"
+
+#ifndef SPIC_NS
+#error "define SPIC_NS and SPIC_STRICT before including spic_binned.inl"
+#endif
+
+namespace spic {
+namespace SPIC_NS {
+
+constexpr bool kStrictB = (SPIC_STRICT != 0);
+
+// Mover cell index 2 + int((p - start) * inv) (src/Mover.cpp:45-47) and
+// deposit cell index 2 + floor(...) (src/Mover.cpp:411-413), in double for
+// FP64 and strict FP32 (the reference promotes to double via xStart),
+// clamped before the conversion so that no input overflows it.
+template <typename T>
+__device__ __forceinline__ int bcell_trunc(T p, double start, T inv, int n) {
+  if (kStrictB || sizeof(T) == 8) {
+    double c = ((double)p - start) * (double)inv;
+    c = fmin(fmax(c, -8.0), (double)n + 8.0);
+    return 2 + (int)c;
+  } else {
+    float c = ((float)p - (float)start) * (float)inv;
+    c = fminf(fmaxf(c, -8.0f), (float)n + 8.0f);
+    return 2 + (int)c;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ int bcell_floor(T p, double start, T inv, int n) {
+  if (kStrictB || sizeof(T) == 8) {
+    double c = ((double)p - start) * (double)inv;
+    c = fmin(fmax(c, -8.0), (double)n + 8.0);
+    return 2 + (int)floor(c);
+  } else {
+    float c = ((float)p - (float)start) * (float)inv;
+    c = fminf(fmaxf(c, -8.0f), (float)n + 8.0f);
+    return 2 + (int)floorf(c);
+  }
+}
+
+// Stencil indices must address nodes 0..n-1: ix-1 >= 0 and ix <= n-1.
+__device__ __forceinline__ bool bclamp(int &ix, int n) {
+  bool bad = (ix < 1) | (ix > n - 1);
+  ix = min(max(ix, 1), n - 1);
+  return bad;
+}
+
+// 1/(1 + omdtsq): in double (src/Mover.cpp:71) for FP64 and strict FP32.
+template <typename T>
+__device__ __forceinline__ T bdenom(T omdtsq) {
+  if (kStrictB || sizeof(T) == 8) return (T)(1.0 / (1.0 + (double)omdtsq));
+  return T(1) / (T(1) + omdtsq);
+}
+
+// The staged block of a tile as the mover's stencil gather sees it: node
+// fields [NB][6] and node coordinates in LDS, and the global node index of
+// block node 0 along each axis.
+template <typename T, int TX, int TY, int TZ>
+struct Block {
+  static constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
+  const T *fld;
+  const T *xn, *yn, *zn;
+  int c0x, c0y, c0z;
+};
+
+// One particle through the predictor-corrector mover and the particle
+// boundary conditions (src/Mover.cpp:33-157), in place.  With STAGED the
+// stencil comes from the block when it lies inside it.
+template <typename T, bool STAGED, int TX, int TY, int TZ>
+__device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T> &f,
+                                              const KPush<T> &k,
+                                              const Block<T, TX, TY, TZ> &blk, T &x,
+                                              T &y, T &z, T &u, T &v, T &w,
+                                              bool &bad, unsigned int &n_lds,
+                                              unsigned int &n_glob) {
+  using B = Block<T, TX, TY, TZ>;
+  const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
+  const T xt = x, yt = y, zt = z;
+  T upt = T(0), vpt = T(0), wpt = T(0);
+  T xc = x, yc = y, zc = z;
+  int cix = -100, ciy = -100, ciz = -100;  // cell of the cached stencil
+  T F[2][2][2][6];
+  T X0 = T(0), X1 = T(0), Y0 = T(0), Y1 = T(0), Z0 = T(0), Z1 = T(0);
+  for (int it = 0; it < k.niter; ++it) {
+    int ix = bcell_trunc<T>(xc, g.xStart, g.invdx, g.nxn);
+    int iy = bcell_trunc<T>(yc, g.yStart, g.invdy, g.nyn);
+    int iz = bcell_trunc<T>(zc, g.zStart, g.invdz, g.nzn);
+    bad |= bclamp(ix, g.nxn);
+    bad |= bclamp(iy, g.nyn);
+    bad |= bclamp(iz, g.nzn);
+    if (ix != cix || iy != ciy || iz != ciz) {
+      cix = ix; ciy = iy; ciz = iz;
+      const int bx = ix - 1 - blk.c0x, by = iy - 1 - blk.c0y, bz = iz - 1 - blk.c0z;
+      const bool in_block = STAGED && (bx >= 0) & (bx <= B::BX - 2) & (by >= 0) &
+                                          (by <= B::BY - 2) & (bz >= 0) & (bz <= B::BZ - 2);
+      if (in_block) {
+        ++n_lds;
+        X0 = blk.xn[bx]; X1 = blk.xn[bx + 1];
+        Y0 = blk.yn[by]; Y1 = blk.yn[by + 1];
+        Z0 = blk.zn[bz]; Z1 = blk.zn[bz + 1];
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              const int nb = ((bx + 1 - aa) * B::BY + (by + 1 - bb)) * B::BZ + (bz + 1 - cc);
+#pragma unroll
+              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = blk.fld[nb * 6 + c];
+            }
+      } else {
+        ++n_glob;
+        X0 = g.xn[ix - 1]; X1 = g.xn[ix];
+        Y0 = g.yn[iy - 1]; Y1 = g.yn[iy];
+        Z0 = g.zn[iz - 1]; Z1 = g.zn[iz];
+        const int gb = (ix * nyn + iy) * nzn + iz;
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              const int n = gb - aa * sxg - bb * nzn - cc;
+              F[aa][bb][cc][0] = f.Ex[n];
+              F[aa][bb][cc][1] = f.Ey[n];
+              F[aa][bb][cc][2] = f.Ez[n];
+              F[aa][bb][cc][3] = f.Bx[n];
+              F[aa][bb][cc][4] = f.By[n];
+              F[aa][bb][cc][5] = f.Bz[n];
+            }
+      }
+    }
+    const T xi[2] = {xc - X0, X1 - xc};
+    const T eta[2] = {yc - Y0, Y1 - yc};
+    const T zeta[2] = {zc - Z0, Z1 - zc};
+    T Exl = T(0), Eyl = T(0), Ezl = T(0), Bxl = T(0), Byl = T(0), Bzl = T(0);
+#pragma unroll
+    for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+          const T wgt = xi[aa] * eta[bb] * zeta[cc] * g.invVOL;
+          Exl += wgt * F[aa][bb][cc][0];
+          Eyl += wgt * F[aa][bb][cc][1];
+          Ezl += wgt * F[aa][bb][cc][2];
+          Bxl += wgt * F[aa][bb][cc][3];
+          Byl += wgt * F[aa][bb][cc][4];
+          Bzl += wgt * F[aa][bb][cc][5];
+        }
+    const T q2 = k.qomdt2;
+    const T omdtsq = q2 * q2 * (Bxl * Bxl + Byl * Byl + Bzl * Bzl);
+    const T denom = bdenom<T>(omdtsq);
+    const T ut = u + q2 * Exl;
+    const T vt = v + q2 * Eyl;
+    const T wt = w + q2 * Ezl;
+    const T udotb = ut * Bxl + vt * Byl + wt * Bzl;
+    upt = (ut + q2 * (vt * Bzl - wt * Byl + q2 * udotb * Bxl)) * denom;
+    vpt = (vt + q2 * (wt * Bxl - ut * Bzl + q2 * udotb * Byl)) * denom;
+    wpt = (wt + q2 * (ut * Byl - vt * Bxl + q2 * udotb * Bzl)) * denom;
+    xc = xt + upt * k.dto2;
+    yc = yt + vpt * k.dto2;
+    zc = zt + wpt * k.dto2;
+  }
+  if (kStrictB || sizeof(T) == 8) {
+    u = (T)(2.0 * (double)upt - (double)u);
+    v = (T)(2.0 * (double)vpt - (double)v);
+    w = (T)(2.0 * (double)wpt - (double)w);
+  } else {
+    u = T(2) * upt - u;
+    v = T(2) * vpt - v;
+    w = T(2) * wpt - w;
+  }
+  x = xt + upt * k.dt_sub;
+  y = yt + vpt * k.dt_sub;
+  z = zt + wpt * k.dt_sub;
+  // particle boundary conditions: two independent tests per axis
+  // (src/Mover.cpp:104-150)
+  if ((double)x > g.Lx) {
+    if (k.perx) x = (T)((double)x - g.Lx);
+    else { u = -u; x = (T)(2 * g.Lx - (double)x); }
+  }
+  if (x < T(0)) {
+    if (k.perx) x = (T)((double)x + g.Lx);
+    else { u = -u; x = -x; }
+  }
+  if ((double)y > g.Ly) {
+    if (k.pery) y = (T)((double)y - g.Ly);
+    else { v = -v; y = (T)(2 * g.Ly - (double)y); }
+  }
+  if (y < T(0)) {
+    if (k.pery) y = (T)((double)y + g.Ly);
+    else { v = -v; y = -y; }
+  }
+  if ((double)z > g.Lz) {
+    if (k.perz) z = (T)((double)z - g.Lz);
+    else { w = -w; z = (T)(2 * g.Lz - (double)z); }
+  }
+  if (z < T(0)) {
+    if (k.perz) z = (T)((double)z + g.Lz);
+    else { w = -w; z = -z; }
+  }
+}
+
+// Tile of the deposit cell (ix, iy, iz), clamped to the physical cells.
+template <typename T>
+__device__ __forceinline__ int tile_of(const BinnedArgs<T> &a, int ix, int iy, int iz) {
+  const int pcx = min(max(ix - 2, 0), a.ncx - 1);
+  const int pcy = min(max(iy - 2, 0), a.ncy - 1);
+  const int pcz = min(max(iz - 2, 0), a.ncz - 1);
+  return ((pcx / a.tx) * a.nty + pcy / a.ty) * a.ntz + pcz / a.tz;
+}
+
+// Output slot of a particle to which its destination tile's counter gave
+// `slot`: in the segment while it has room, else the next slot of the
+// overflow region; -1 if that is full too.
+template <typename T>
+__device__ __forceinline__ int64_t out_slot(const BinnedArgs<T> &a,
+                                            const BinnedSpecies<T> &sp, int dest,
+                                            unsigned int slot) {
+  const int64_t sb = sp.seg[dest];
+  if ((int64_t)slot < sp.seg[dest + 1] - sb) return sb + slot;
+  const unsigned int k = atomicAdd(sp.cnt_out + a.ntiles, 1u);
+  return k < sp.ovf_cap ? sp.seg[a.ntiles] + (int64_t)k : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, int64_t o,
+                                               T x, T y, T z, T u, T v, T w, T q,
+                                               uint32_t pid) {
+  sp.out[0][o] = x; sp.out[1][o] = y; sp.out[2][o] = z;
+  sp.out[3][o] = u; sp.out[4][o] = v; sp.out[5][o] = w;
+  sp.out[6][o] = q;
+  sp.id_out[o] = pid;
+}
+
+// The ten moment terms of one node weight, in the reference's operation
+// order (src/Mover.cpp:430-568): rho = wgt*iv, Jx = (u*wgt)*iv, ...,
+// pzz = ((w*w)*wgt)*iv, each converted to A and added to acc.
+template <typename T, typename A>
+__device__ __forceinline__ void add_terms(A *acc, T wgt, T iv, T u, T v, T w, T uu,
+                                          T uv, T uw, T vv, T vw, T ww) {
+  acc[0] += (A)(wgt * iv);
+  acc[1] += (A)((u * wgt) * iv);
+  acc[2] += (A)((v * wgt) * iv);
+  acc[3] += (A)((w * wgt) * iv);
+  acc[4] += (A)((uu * wgt) * iv);
+  acc[5] += (A)((uv * wgt) * iv);
+  acc[6] += (A)((uw * wgt) * iv);
+  acc[7] += (A)((vv * wgt) * iv);
+  acc[8] += (A)((vw * wgt) * iv);
+  acc[9] += (A)((ww * wgt) * iv);
+}
+
+// All 80 contributions of one particle with global FP64 atomics.
+template <typename T>
+__device__ __forceinline__ void deposit_global(const KGrid<T> &g, const KMom &m, int ix,
+                                               int iy, int iz, T x, T y, T z, T u, T v,
+                                               T w, T q) {
+  const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
+  const T xi[2] = {x - g.xn[ix - 1], g.xn[ix] - x};
+  const T eta[2] = {y - g.yn[iy - 1], g.yn[iy] - y};
+  const T zeta[2] = {z - g.zn[iz - 1], g.zn[iz] - z};
+  const T iv = g.invVOL;
+  const T uu = u * u, uv = u * v, uw = u * w, vv = v * v, vw = v * w, ww = w * w;
+  const int gb = (ix * nyn + iy) * nzn + iz;
+#pragma unroll
+  for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const T wgt = q * xi[aa] * eta[bb] * zeta[cc] * iv;
+        double t[10];
+#pragma unroll
+        for (int mm = 0; mm < 10; ++mm) t[mm] = 0.0;
+        add_terms<T, double>(t, wgt, iv, u, v, w, uu, uv, uw, vv, vw, ww);
+        const int n = gb - aa * sxg - bb * nzn - cc;
+#pragma unroll
+        for (int mm = 0; mm < 10; ++mm) unsafeAtomicAdd(m.m[mm] + n, t[mm]);
+      }
+}
+
+// One node (corner aa, bb, cc) of one particle's deposit: its ten terms
+// added to the LDS block when the particle's stencil lies inside the
+// staged block, else to the global FP64 arrays.
+template <typename T, typename A, int BX, int BY, int BZ>
+__device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A *s_mom,
+                                             const T *s_xn, const T *s_yn, const T *s_zn,
+                                             int c0x, int c0y, int c0z, int ix, int iy,
+                                             int iz, int aa, int bb, int cc, T x, T y,
+                                             T z, T u, T v, T w, T q) {
+  constexpr int NB = BX * BY * BZ;
+  const int bx = ix - 1 - c0x, by = iy - 1 - c0y, bz = iz - 1 - c0z;
+  const bool in_block = (bx >= 0) & (bx <= BX - 2) & (by >= 0) & (by <= BY - 2) &
+                        (bz >= 0) & (bz <= BZ - 2);
+  T xi, eta, zeta;
+  if (in_block) {
+    xi = aa == 0 ? x - s_xn[bx] : s_xn[bx + 1] - x;
+    eta = bb == 0 ? y - s_yn[by] : s_yn[by + 1] - y;
+    zeta = cc == 0 ? z - s_zn[bz] : s_zn[bz + 1] - z;
+  } else {
+    xi = aa == 0 ? x - g.xn[ix - 1] : g.xn[ix] - x;
+    eta = bb == 0 ? y - g.yn[iy - 1] : g.yn[iy] - y;
+    zeta = cc == 0 ? z - g.zn[iz - 1] : g.zn[iz] - z;
+  }
+  const T iv = g.invVOL;
+  const T wgt = q * xi * eta * zeta * iv;
+  A t[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) t[k] = A(0);
+  add_terms<T, A>(t, wgt, iv, u, v, w, u * u, u * v, u * w, v * v, v * w, w * w);
+  if (in_block) {
+    const int nb = ((bx + 1 - aa) * BY + (by + 1 - bb)) * BZ + (bz + 1 - cc);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) atomicAdd(&s_mom[k * NB + nb], t[k]);
+  } else {
+    const int n = ((ix - aa) * g.nyn + (iy - bb)) * g.nzn + (iz - cc);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) unsafeAtomicAdd(m.m[k] + n, (double)t[k]);
+  }
+  return in_block;
+}
+
+// A: type of the LDS moment block and register sums (T, or double for
+// FP32 particles).
+template <typename T, typename A, int TX, int TY, int TZ>
+__global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T> a) {
+  using Blk = Block<T, TX, TY, TZ>;
+  constexpr int NC = TX * TY * TZ;  // cells per tile
+  constexpr int NT = 4 * NC;        // threads: four per cell
+  constexpr int BX = Blk::BX, BY = Blk::BY, BZ = Blk::BZ, NB = Blk::NB;
+  constexpr int CAP = kBucketCap;
+  // bucket stride per cell: odd, so that the 16 cells a wave reads in the
+  // bucket step fall on distinct LDS banks
+  constexpr int CS = CAP | 1;
+  constexpr int NL = kListCap;
+  __shared__ T s_fld[NB * 6];
+  __shared__ A s_mom[10 * NB];
+  __shared__ T s_xn[BX], s_yn[BY], s_zn[BZ];
+  __shared__ T s_bkt[7][NC * CS];  // x y z q u v w of the bucketed particles
+  __shared__ T s_lst[7][NL];       // x y z q u v w of the listed particles
+  __shared__ unsigned int s_cnt[2][NC];
+  __shared__ unsigned int s_nl[2];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int tile = blockIdx.x;
+  const int tz = tile % a.ntz, ty = (tile / a.ntz) % a.nty, tx = tile / (a.ntz * a.nty);
+  // block node b along x is global node c0x + b; cell c of the tile uses
+  // block nodes c+1 and c+2
+  Blk blk;
+  blk.fld = s_fld;
+  blk.xn = s_xn; blk.yn = s_yn; blk.zn = s_zn;
+  blk.c0x = tx * TX; blk.c0y = ty * TY; blk.c0z = tz * TZ;
+  const int nyn = a.g.nyn, nzn = a.g.nzn;
+
+  // 1. stage node coordinates and fields
+  for (int b = tid; b < BX + BY + BZ; b += NT) {
+    if (b < BX) s_xn[b] = (blk.c0x + b < a.g.nxn) ? a.g.xn[blk.c0x + b] : T(0);
+    else if (b < BX + BY)
+      s_yn[b - BX] = (blk.c0y + b - BX < nyn) ? a.g.yn[blk.c0y + b - BX] : T(0);
+    else
+      s_zn[b - BX - BY] =
+          (blk.c0z + b - BX - BY < nzn) ? a.g.zn[blk.c0z + b - BX - BY] : T(0);
+  }
+  for (int b = tid; b < NB; b += NT) {
+    const int bz = b % BZ, by = (b / BZ) % BY, bx = b / (BZ * BY);
+    const int gx = blk.c0x + bx, gy = blk.c0y + by, gz = blk.c0z + bz;
+    T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (gx < a.g.nxn && gy < nyn && gz < nzn) {
+      const int n = (gx * nyn + gy) * nzn + gz;
+      v[0] = a.f.Ex[n]; v[1] = a.f.Ey[n]; v[2] = a.f.Ez[n];
+      v[3] = a.f.Bx[n]; v[4] = a.f.By[n]; v[5] = a.f.Bz[n];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s_fld[b * 6 + c] = v[c];
+  }
+
+  // this thread's cell and node pair in the bucket step: corners (a, b, 0)
+  // and (a, b, 1) of cell (mlx, mly, mlz)
+  const int my_cell = tid >> 2;
+  const int my_a = (tid >> 1) & 1, my_b = tid & 1;
+  const int mlx = my_cell / (TY * TZ), mly = (my_cell / TZ) % TY, mlz = my_cell % TZ;
+
+  bool bad_move = false, bad_dep = false, bad_store = false;
+  unsigned int n_gg = 0, n_gd = 0, n_lg = 0, n_ld = 0;
+
+  for (int s = 0; s < a.ns; ++s) {
+    const BinnedSpecies<T> &sp = a.sp[s];
+    const int64_t p0 = sp.seg[tile];
+    const int64_t nin = min((int64_t)sp.cnt_in[tile], sp.seg[tile + 1] - p0);
+    if (nin == 0) continue;  // uniform across the workgroup
+    for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
+    for (int i = tid; i < 2 * NC; i += NT) (&s_cnt[0][0])[i] = 0u;
+    if (tid < 2) s_nl[tid] = 0u;
+    __syncthreads();
+    A acc[20];
+#pragma unroll
+    for (int k = 0; k < 20; ++k) acc[k] = A(0);
+
+    const int64_t nbatches = (nin + NT - 1) / NT;
+    for (int64_t j = 0; j < nbatches; ++j) {
+      const int64_t r = j * NT + tid;
+      const bool active = r < nin;
+      const int buf = (int)(j & 1);
+      T x = T(0), y = T(0), z = T(0), u = T(0), v = T(0), w = T(0), q = T(0);
+      uint32_t pid = 0;
+      if (active) {
+        const int64_t i = p0 + r;
+        x = sp.in[0][i]; y = sp.in[1][i]; z = sp.in[2][i];
+        u = sp.in[3][i]; v = sp.in[4][i]; w = sp.in[5][i]; q = sp.in[6][i];
+        pid = sp.id_in[i];
+      }
+      int ix = 2, iy = 2, iz = 2, dest = -1;
+      if (active) {
+        push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
+                                           bad_move, n_lg, n_gg);
+        ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
+        iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
+        iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
+        bad_dep |= bclamp(ix, a.g.nxn);
+        bad_dep |= bclamp(iy, a.g.nyn);
+        bad_dep |= bclamp(iz, a.g.nzn);
+        dest = tile_of(a, ix, iy, iz);
+      }
+
+      // re-bin: one counter atomic per wave for the particles that stay
+      {
+        const bool stay = active && dest == tile;
+        const unsigned long long stay_mask = __ballot(stay);
+        unsigned int slot = 0;
+        if (stay_mask) {
+          const int leader = __ffsll((long long)stay_mask) - 1;
+          unsigned int wbase = 0;
+          if (lane == leader)
+            wbase = atomicAdd(sp.cnt_out + tile, (unsigned int)__popcll(stay_mask));
+          wbase = __shfl(wbase, leader);
+          if (stay)
+            slot = wbase + (unsigned int)__popcll(stay_mask & ((1ull << lane) - 1ull));
+        }
+        if (active && !stay) slot = atomicAdd(sp.cnt_out + dest, 1u);
+        if (active) {
+          const int64_t o = out_slot(a, sp, dest, slot);
+          if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
+          else bad_store = true;
+        }
+      }
+
+      // deposit: bucket (tile cell), LDS atomics (halo cell), global
+      if (active) {
+        const int lcx = ix - 2 - blk.c0x, lcy = iy - 2 - blk.c0y, lcz = iz - 2 - blk.c0z;
+        const bool in_tile = (lcx >= 0) & (lcx < TX) & (lcy >= 0) & (lcy < TY) &
+                             (lcz >= 0) & (lcz < TZ);
+        bool done = false;
+        if (in_tile) {
+          const int cell = (lcx * TY + lcy) * TZ + lcz;
+          const unsigned int sl = atomicAdd(&s_cnt[buf][cell], 1u);
+          if (sl < (unsigned int)CAP) {
+            const int e = cell * CS + (int)sl;
+            s_bkt[0][e] = x; s_bkt[1][e] = y; s_bkt[2][e] = z; s_bkt[3][e] = q;
+            s_bkt[4][e] = u; s_bkt[5][e] = v; s_bkt[6][e] = w;
+            done = true;
+            ++n_ld;
+          }
+        }
+        if (!done) {
+          // halo cell, outside the block or full bucket: to the list
+          const unsigned int k = atomicAdd(&s_nl[buf], 1u);
+          if (k < (unsigned int)NL) {
+            s_lst[0][k] = x; s_lst[1][k] = y; s_lst[2][k] = z; s_lst[3][k] = q;
+            s_lst[4][k] = u; s_lst[5][k] = v; s_lst[6][k] = w;
+            done = true;
+          }
+        }
+        if (!done) {
+          // list full: this thread deposits all eight nodes itself
+          bool lds = false;
+#pragma unroll 1
+          for (int nd = 0; nd < 8; ++nd)
+            lds = deposit_node<T, A, BX, BY, BZ>(a.g, sp.m, s_mom, s_xn, s_yn, s_zn,
+                                                 blk.c0x, blk.c0y, blk.c0z, ix, iy, iz,
+                                                 nd >> 2, (nd >> 1) & 1, nd & 1, x, y, z,
+                                                 u, v, w, q);
+          if (lds) ++n_ld;
+          else ++n_gd;
+        }
+      }
+
+      // bucket step: each cell's four threads sum the cell's bucket
+      __syncthreads();
+      {
+        const unsigned int nb_c = min(s_cnt[buf][my_cell], (unsigned int)CAP);
+        if ((tid & 3) == 0) s_cnt[buf ^ 1][my_cell] = 0u;  // for the next batch
+        const T X0 = s_xn[mlx + 1], X1 = s_xn[mlx + 2];
+        const T Y0 = s_yn[mly + 1], Y1 = s_yn[mly + 2];
+        const T Z0 = s_zn[mlz + 1], Z1 = s_zn[mlz + 2];
+        const T iv = a.g.invVOL;
+        for (unsigned int k = 0; k < nb_c; ++k) {
+          const int e = my_cell * CS + (int)k;
+          const T px = s_bkt[0][e], py = s_bkt[1][e], pz = s_bkt[2][e];
+          const T pq = s_bkt[3][e], pu = s_bkt[4][e], pv = s_bkt[5][e], pw = s_bkt[6][e];
+          const T xa = my_a == 0 ? px - X0 : X1 - px;
+          const T yb = my_b == 0 ? py - Y0 : Y1 - py;
+          const T uu = pu * pu, uv = pu * pv, uw = pu * pw, vv = pv * pv, vw = pv * pw,
+                  ww = pw * pw;
+          if (kStrictB) {
+            // the reference's terms: (p_m * (q xi eta zeta iv)) * iv
+            const T w0 = pq * xa * yb * (pz - Z0) * iv;  // corner (a, b, 0)
+            const T w1 = pq * xa * yb * (Z1 - pz) * iv;  // corner (a, b, 1)
+            add_terms<T, A>(acc, w0, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
+            add_terms<T, A>(acc + 10, w1, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
+          } else {
+            // sum of p_m * (q xi eta zeta); the factor iv^2 is applied once
+            // when the sums are added to the LDS block
+            const T qxy = pq * xa * yb;
+            const A w0 = (A)(qxy * (pz - Z0)), w1 = (A)(qxy * (Z1 - pz));
+            const A pm[10] = {A(1), (A)pu, (A)pv, (A)pw, (A)uu,
+                              (A)uv, (A)uw, (A)vv, (A)vw, (A)ww};
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+              acc[m] += pm[m] * w0;
+              acc[10 + m] += pm[m] * w1;
+            }
+          }
+        }
+      }
+      // list step: eight threads per listed particle, one node each
+      {
+        const unsigned int nl = min(s_nl[buf], (unsigned int)NL);
+        const int nd = tid & 7;
+        for (unsigned int k = (unsigned int)(tid >> 3); k < nl; k += NT / 8) {
+          const T px = s_lst[0][k], py = s_lst[1][k], pz = s_lst[2][k];
+          const T pq = s_lst[3][k], pu = s_lst[4][k], pv = s_lst[5][k], pw = s_lst[6][k];
+          // the inserting thread's deposit cell, recomputed
+          int jx = bcell_floor<T>(px, a.g.xStart, a.g.invdx, a.g.nxn);
+          int jy = bcell_floor<T>(py, a.g.yStart, a.g.invdy, a.g.nyn);
+          int jz = bcell_floor<T>(pz, a.g.zStart, a.g.invdz, a.g.nzn);
+          bclamp(jx, a.g.nxn);
+          bclamp(jy, a.g.nyn);
+          bclamp(jz, a.g.nzn);
+          const bool lds = deposit_node<T, A, BX, BY, BZ>(
+              a.g, sp.m, s_mom, s_xn, s_yn, s_zn, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz,
+              nd >> 2, (nd >> 1) & 1, nd & 1, px, py, pz, pu, pv, pw, pq);
+          if (nd == 0) {
+            if (lds) ++n_ld;
+            else ++n_gd;
+          }
+        }
+        if (tid == 0) s_nl[buf ^ 1] = 0u;  // for the next batch
+      }
+      __syncthreads();
+    }
+
+    // 3. register sums into the LDS block: corner (a, b, c) of cell
+    // (lx, ly, lz) is block node (lx + 2 - a, ly + 2 - b, lz + 2 - c)
+    const A scale = kStrictB ? A(1) : (A)a.g.invVOL * (A)a.g.invVOL;
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int nb = ((mlx + 2 - my_a) * BY + (mly + 2 - my_b)) * BZ + (mlz + 2 - cc);
+#pragma unroll
+      for (int m = 0; m < 10; ++m)
+        if (acc[cc * 10 + m] != A(0))
+          atomicAdd(&s_mom[m * NB + nb], acc[cc * 10 + m] * scale);
+    }
+    __syncthreads();
+    double *dst = a.partials + ((size_t)tile * a.ns + s) * (10 * NB);
+    for (int i = tid; i < 10 * NB; i += NT) dst[i] = (double)s_mom[i];
+    __syncthreads();
+  }
+  if (bad_move) atomicAdd(a.status, 1u);
+  if (bad_dep) atomicAdd(a.status + 1, 1u);
+  if (bad_store) atomicAdd(a.status + 2, 1u);
+  if (a.counters) {
+    if (n_gg) atomicAdd(a.counters + 0, (unsigned long long)n_gg);
+    if (n_gd) atomicAdd(a.counters + 1, (unsigned long long)n_gd);
+    if (n_lg) atomicAdd(a.counters + 2, (unsigned long long)n_lg);
+    if (n_ld) atomicAdd(a.counters + 3, (unsigned long long)n_ld);
+  }
+}
+
+// The particles of every species' input overflow region: global gathers
+// and deposits, re-binned like the tile kernel's.
+template <typename T>
+__global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
+  Block<T, 1, 1, 1> none;
+  none.fld = nullptr;
+  none.xn = none.yn = none.zn = nullptr;
+  none.c0x = none.c0y = none.c0z = 0;
+  bool bad_move = false, bad_dep = false, bad_store = false;
+  unsigned int n_gg = 0, n_gd = 0, n_lg = 0;
+  for (int s = 0; s < a.ns; ++s) {
+    const BinnedSpecies<T> &sp = a.sp[s];
+    const int64_t p0 = sp.seg[a.ntiles];
+    const int64_t nin = min((int64_t)sp.cnt_in[a.ntiles], (int64_t)sp.ovf_cap);
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nin;
+         r += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t i = p0 + r;
+      T x = sp.in[0][i], y = sp.in[1][i], z = sp.in[2][i];
+      T u = sp.in[3][i], v = sp.in[4][i], w = sp.in[5][i];
+      const T q = sp.in[6][i];
+      const uint32_t pid = sp.id_in[i];
+      push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
+                                       bad_move, n_lg, n_gg);
+      int ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
+      int iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
+      int iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
+      bad_dep |= bclamp(ix, a.g.nxn);
+      bad_dep |= bclamp(iy, a.g.nyn);
+      bad_dep |= bclamp(iz, a.g.nzn);
+      const int dest = tile_of(a, ix, iy, iz);
+      const int64_t o = out_slot(a, sp, dest, atomicAdd(sp.cnt_out + dest, 1u));
+      if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
+      else bad_store = true;
+      ++n_gd;
+      deposit_global<T>(a.g, sp.m, ix, iy, iz, x, y, z, u, v, w, q);
+    }
+  }
+  if (bad_move) atomicAdd(a.status, 1u);
+  if (bad_dep) atomicAdd(a.status + 1, 1u);
+  if (bad_store) atomicAdd(a.status + 2, 1u);
+  if (a.counters) {
+    if (n_gg) atomicAdd(a.counters + 0, (unsigned long long)n_gg);
+    if (n_gd) atomicAdd(a.counters + 1, (unsigned long long)n_gd);
+  }
+}
+
+template <typename T, typename A>
+hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
+  if (a.tx == 4 && a.ty == 4 && a.tz == 4)
+    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4>), dim3(a.ntiles), dim3(256), 0,
+                       s, a);
+  else if (a.tx == 8 && a.ty == 4 && a.tz == 4)
+    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4>), dim3(a.ntiles), dim3(512), 0,
+                       s, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
+  if (a.ntiles <= 0 || a.ns <= 0) return hipSuccess;
+  hipError_t e = (sizeof(T) == 4 && a.lds_fp64) ? launch_binned_shape<T, double>(a, s)
+                                                : launch_binned_shape<T, T>(a, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_binned_overflow<T>, dim3(a.n_ovf_blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);
+template hipError_t launch_binned_phase<float>(const BinnedArgs<float> &, hipStream_t);
+
+}  // namespace SPIC_NS
+}  // namespace spic

@@ -1,0 +1,353 @@
+// spic_binsort.hip -- binned particle storage: the sort that (re)builds it,
+// the store back to the caller's order, the per-node reduction of the
+// phase kernel's tile partials, and the count reset before each phase.
+//
+// Sort.  A particle's cell is its deposit cell 2 + floor((p - start) * inv)
+// minus the two guard offsets, clamped to the physical cells; its rank is
+// a per-cell counter taken with an atomic.  The key
+//   tile | rank within the cell | cell within the tile
+// orders each tile's particles round-robin over its cells (every cell's
+// first particle, then every cell's second, ...), so that any run of
+// 4 * cells-per-tile consecutive particles holds about four per cell: the
+// phase kernel takes one such run per batch into its per-cell buckets, and
+// particles that stay in their tile keep this order from phase to phase.
+// A stable LSD radix sort (rocPRIM) of (key, source slot) pairs gives the
+// permutation; the seven arrays and the particle id are gathered to sorted
+// position i + tile * slack, so that every segment gets `slack` free slots
+// after its particles.  A binned source is enumerated densely through an
+// exclusive scan of its counts.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "spic_binned.hpp"
+
+namespace spic {
+namespace {
+
+inline unsigned nblocks(long n, int tpb = 256) {
+  long b = (n + tpb - 1) / tpb;
+  if (b < 1) b = 1;
+  if (b > (1L << 20)) b = 1L << 20;
+  return (unsigned)b;
+}
+
+inline size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+template <typename T>
+__device__ __forceinline__ int phys_cell(T p, double start, T inv, int nc) {
+  double c = floor(((double)p - start) * (double)inv);
+  c = fmin(fmax(c, 0.0), (double)(nc - 1));
+  return (int)c;
+}
+
+// Particles of a binned source that a dense enumeration can reach: the
+// count of each segment clamped to its capacity (the excess was written to
+// the overflow region), and of the overflow region to its capacity.
+__global__ void k_clamped_counts(const int64_t *__restrict__ seg,
+                                 const unsigned int *__restrict__ cnt,
+                                 unsigned int ovf_cap, int ntiles,
+                                 unsigned int *__restrict__ out) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
+       t += gridDim.x * blockDim.x) {
+    const int64_t cap = t < ntiles ? seg[t + 1] - seg[t] : (int64_t)ovf_cap;
+    out[t] = (unsigned int)min((int64_t)cnt[t], cap);
+  }
+}
+
+// Source slot of dense particle i: the last range t with P[t] <= i.
+__device__ __forceinline__ int64_t source_slot(int64_t i, const unsigned int *P,
+                                               const int64_t *seg, int ntiles) {
+  int lo = 0, hi = ntiles;  // answer in [0, ntiles]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)P[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return seg[lo] + (i - (int64_t)P[lo]);
+}
+
+template <typename T>
+__global__ void k_bin_keys(KGrid<T> g, BinLayout L, int64_t n,
+                           const T *__restrict__ x, const T *__restrict__ y,
+                           const T *__restrict__ z,
+                           const unsigned int *__restrict__ P,
+                           const int64_t *__restrict__ seg,
+                           unsigned int *__restrict__ cell_count,
+                           uint32_t *__restrict__ key, uint32_t *__restrict__ idx) {
+  const uint32_t rank_max = (1u << L.rank_bits) - 1u;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = P ? source_slot(i, P, seg, L.ntiles) : i;
+    const int cx = phys_cell<T>(x[j], g.xStart, g.invdx, L.ncx);
+    const int cy = phys_cell<T>(y[j], g.yStart, g.invdy, L.ncy);
+    const int cz = phys_cell<T>(z[j], g.zStart, g.invdz, L.ncz);
+    const int tx = cx / L.tx, ty = cy / L.ty, tz = cz / L.tz;
+    const uint32_t tile = (uint32_t)((tx * L.nty + ty) * L.ntz + tz);
+    const uint32_t cit = (uint32_t)(((cx - tx * L.tx) * L.ty + (cy - ty * L.ty)) * L.tz +
+                                    (cz - tz * L.tz));
+    uint32_t r = atomicAdd(&cell_count[((size_t)cx * L.ncy + cy) * L.ncz + cz], 1u);
+    r = r < rank_max ? r : rank_max;
+    key[i] = (tile << (L.rank_bits + L.cell_bits)) | (r << L.cell_bits) | cit;
+    idx[i] = (uint32_t)j;
+  }
+}
+
+__global__ void k_zero_u32(unsigned int *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+template <typename T>
+struct Arrays7 {
+  T *p[7];
+};
+
+template <typename T>
+__global__ void k_bin_gather(Arrays7<T> src, const uint32_t *__restrict__ src_id,
+                             const uint32_t *__restrict__ perm,
+                             const uint32_t *__restrict__ keys, int shift, int64_t n,
+                             int64_t slack, Arrays7<T> dst,
+                             uint32_t *__restrict__ dst_id) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t j = perm[i];
+    const int64_t o = i + (int64_t)(keys[i] >> shift) * slack;
+#pragma unroll
+    for (int a = 0; a < 7; ++a) dst.p[a][o] = src.p[a][j];
+    dst_id[o] = src_id ? src_id[j] : j;
+  }
+}
+
+// seg[t] = (first sorted position of tile >= t) + t * slack; cnt from the
+// differences; the overflow region starts empty.
+__global__ void k_bin_bounds(const uint32_t *__restrict__ keys, int shift, int64_t n,
+                             int ntiles, int64_t slack, int64_t *__restrict__ seg,
+                             unsigned int *__restrict__ cnt) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
+       t += gridDim.x * blockDim.x) {
+    int64_t b[2];
+    for (int k = 0; k < 2; ++k) {
+      const int64_t tt = t + k;
+      int64_t lo = 0, hi = n;
+      while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if ((int64_t)(keys[mid] >> shift) < tt) lo = mid + 1;
+        else hi = mid;
+      }
+      b[k] = lo;
+    }
+    seg[t] = b[0] + (int64_t)t * slack;
+    cnt[t] = t < ntiles ? (unsigned int)(b[1] - b[0]) : 0u;
+  }
+}
+
+// One workgroup per segment (then the overflow region): dst[id] = src.
+template <typename T>
+__global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
+                            const int64_t *__restrict__ seg,
+                            const unsigned int *__restrict__ cnt,
+                            unsigned int ovf_cap, int ntiles, Arrays7<T> dst) {
+  const int t = blockIdx.x;
+  const int64_t p0 = seg[t];
+  const int64_t cap = t < ntiles ? seg[t + 1] - p0 : (int64_t)ovf_cap;
+  const int64_t m = min((int64_t)cnt[t], cap);
+  for (int64_t r = threadIdx.x; r < m; r += blockDim.x) {
+    const int64_t i = p0 + r;
+    const uint32_t k = id[i];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) dst.p[a][k] = src.p[a][i];
+  }
+}
+
+__global__ void k_zero_counts(CountArrays c) {
+  const int s = blockIdx.y;
+  if (s >= c.ns) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n;
+       i += gridDim.x * blockDim.x)
+    c.p[s][i] = 0u;
+}
+
+// Per node and species: the sum of the partials of every tile whose staged
+// block holds the node (at most two tiles per axis), in tile order, with
+// Neumaier's compensated summation, added to the FP64 moment arrays.
+// Tiles whose input segment held no particle of the species wrote no
+// partials and are skipped.
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int nn) {
+  constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
+  const int s = blockIdx.y;
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  const int nyn = a.nyn, nzn = a.nzn;
+  const int gz = n % nzn, gy = (n / nzn) % nyn, gx = n / (nzn * nyn);
+  const int xl = gx >= BX - 1 ? (gx - (BX - 1) + TX - 1) / TX : 0;
+  const int xh = min(gx / TX, a.ntx - 1);
+  const int yl = gy >= BY - 1 ? (gy - (BY - 1) + TY - 1) / TY : 0;
+  const int yh = min(gy / TY, a.nty - 1);
+  const int zl = gz >= BZ - 1 ? (gz - (BZ - 1) + TZ - 1) / TZ : 0;
+  const int zh = min(gz / TZ, a.ntz - 1);
+  const unsigned int *cnt = a.cnt[s];
+  double sum[10], cmp[10];
+#pragma unroll
+  for (int m = 0; m < 10; ++m) { sum[m] = 0.0; cmp[m] = 0.0; }
+  bool any = false;
+  for (int tx = xl; tx <= xh; ++tx)
+    for (int ty = yl; ty <= yh; ++ty)
+      for (int tz = zl; tz <= zh; ++tz) {
+        const int tile = (tx * a.nty + ty) * a.ntz + tz;
+        if (cnt[tile] == 0u) continue;
+        any = true;
+        const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
+        const double *src = a.partials + ((size_t)tile * a.ns + s) * (10 * NB) + b;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+          const double v = src[m * NB];
+          const double t = sum[m] + v;
+          cmp[m] += fabs(sum[m]) >= fabs(v) ? (sum[m] - t) + v : (v - t) + sum[m];
+          sum[m] = t;
+        }
+      }
+  if (!any) return;
+#pragma unroll
+  for (int m = 0; m < 10; ++m) a.mom[s].m[m][n] += sum[m] + cmp[m];
+}
+
+}  // namespace
+
+hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int tz,
+                                int nn, hipStream_t s) {
+  if (nn <= 0 || a.ns <= 0) return hipSuccess;
+  dim3 grid((nn + 255) / 256, a.ns);
+  if (tx == 4 && ty == 4 && tz == 4)
+    hipLaunchKernelGGL((k_binned_reduce<4, 4, 4>), grid, dim3(256), 0, s, a, nn);
+  else if (tx == 8 && ty == 4 && tz == 4)
+    hipLaunchKernelGGL((k_binned_reduce<8, 4, 4>), grid, dim3(256), 0, s, a, nn);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
+  if (c.ns <= 0 || c.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n), c.ns), dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
+                    T *const src[7], const uint32_t *src_id,
+                    const int64_t *src_seg, const unsigned int *src_cnt,
+                    unsigned int src_ovf_cap, T *const dst[7], uint32_t *dst_id,
+                    int64_t *dst_seg, unsigned int *dst_cnt, void *workspace,
+                    size_t workspace_bytes, size_t *needed, hipStream_t s) {
+  if (n < 0 || n > 0xFFFFFFFFll) return hipErrorInvalidValue;
+  const bool binned = src_seg != nullptr;
+  const size_t nt1 = (size_t)L.ntiles + 1;
+  const size_t a_n = align_up(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  const size_t a_t = align_up(sizeof(uint32_t) * nt1);
+  const int64_t ncell = (int64_t)L.ncx * L.ncy * L.ncz;
+  const size_t a_c = align_up(sizeof(uint32_t) * (size_t)ncell);
+  const int shift = L.rank_bits + L.cell_bits;
+  const int key_bits = L.tile_bits + shift;
+  size_t sort_bytes = 0, scan_bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(
+      nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+      (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0, key_bits, s);
+  if (e != hipSuccess) return e;
+  e = rocprim::exclusive_scan(nullptr, scan_bytes, (const unsigned int *)nullptr,
+                              (unsigned int *)nullptr, 0u, nt1,
+                              rocprim::plus<unsigned int>(), s);
+  if (e != hipSuccess) return e;
+  const size_t total =
+      4 * a_n + 2 * a_t + a_c + align_up(sort_bytes) + align_up(scan_bytes);
+  if (needed) *needed = total;
+  if (!workspace) return hipSuccess;
+  if (workspace_bytes < total) return hipErrorInvalidValue;
+  char *w = (char *)workspace;
+  uint32_t *keys_in = (uint32_t *)w;
+  uint32_t *keys_out = (uint32_t *)(w + a_n);
+  uint32_t *idx_in = (uint32_t *)(w + 2 * a_n);
+  uint32_t *perm = (uint32_t *)(w + 3 * a_n);
+  unsigned int *clamped = (unsigned int *)(w + 4 * a_n);
+  unsigned int *P = (unsigned int *)(w + 4 * a_n + a_t);
+  unsigned int *cell_count = (unsigned int *)(w + 4 * a_n + 2 * a_t);
+  void *sort_tmp = (void *)(w + 4 * a_n + 2 * a_t + a_c);
+  void *scan_tmp = (void *)(w + 4 * a_n + 2 * a_t + a_c + align_up(sort_bytes));
+
+  if (binned && n > 0) {
+    hipLaunchKernelGGL(k_clamped_counts, dim3(nblocks((long)nt1)), dim3(256), 0, s,
+                       src_seg, src_cnt, src_ovf_cap, L.ntiles, clamped);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = rocprim::exclusive_scan(scan_tmp, scan_bytes, (const unsigned int *)clamped,
+                                P, 0u, nt1, rocprim::plus<unsigned int>(), s);
+    if (e != hipSuccess) return e;
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(ncell)), dim3(256), 0, s, cell_count,
+                       ncell);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_keys<T>, dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
+                       (const T *)src[0], (const T *)src[1], (const T *)src[2],
+                       binned ? (const unsigned int *)P : nullptr, src_seg, cell_count,
+                       keys_in, idx_in);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, idx_in,
+                                  perm, (size_t)n, 0, key_bits, s);
+    if (e != hipSuccess) return e;
+    Arrays7<T> as, ad;
+    for (int a = 0; a < 7; ++a) {
+      as.p[a] = src[a];
+      ad.p[a] = dst[a];
+    }
+    hipLaunchKernelGGL(k_bin_gather<T>, dim3(nblocks(n)), dim3(256), 0, s, as,
+                       binned ? src_id : (const uint32_t *)nullptr,
+                       (const uint32_t *)perm, (const uint32_t *)keys_out, shift, n,
+                       L.slack, ad, dst_id);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_bin_bounds, dim3(nblocks((long)nt1)), dim3(256), 0, s,
+                     (const uint32_t *)keys_out, shift, n, L.ntiles, L.slack, dst_seg,
+                     dst_cnt);
+  return hipGetLastError();
+}
+
+template hipError_t bin_sort<double>(const KGrid<double> &, const BinLayout &, int64_t,
+                                     double *const[7], const uint32_t *,
+                                     const int64_t *, const unsigned int *,
+                                     unsigned int, double *const[7], uint32_t *,
+                                     int64_t *, unsigned int *, void *, size_t,
+                                     size_t *, hipStream_t);
+template hipError_t bin_sort<float>(const KGrid<float> &, const BinLayout &, int64_t,
+                                    float *const[7], const uint32_t *,
+                                    const int64_t *, const unsigned int *,
+                                    unsigned int, float *const[7], uint32_t *,
+                                    int64_t *, unsigned int *, void *, size_t,
+                                    size_t *, hipStream_t);
+
+template <typename T>
+hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
+                     const int64_t *seg, const unsigned int *cnt,
+                     unsigned int ovf_cap, T *const dst[7], hipStream_t s) {
+  Arrays7<T> as, ad;
+  for (int a = 0; a < 7; ++a) {
+    as.p[a] = src[a];
+    ad.p[a] = dst[a];
+  }
+  hipLaunchKernelGGL(k_bin_store<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
+                     seg, cnt, ovf_cap, L.ntiles, ad);
+  return hipGetLastError();
+}
+
+template hipError_t bin_store<double>(const BinLayout &, double *const[7],
+                                      const uint32_t *, const int64_t *,
+                                      const unsigned int *, unsigned int,
+                                      double *const[7], hipStream_t);
+template hipError_t bin_store<float>(const BinLayout &, float *const[7],
+                                     const uint32_t *, const int64_t *,
+                                     const unsigned int *, unsigned int,
+                                     float *const[7], hipStream_t);
+
+}  // namespace spic

@@ -52,7 +52,7 @@ using spic::kpush;
 
 namespace spic {
 // Per-thread, per-device status words: [0] mover iterates out of the grid,
-// [1] deposit cells out of the grid.  Thread-local so that concurrent
+// [1] deposit cells out of the grid, [2] engine particle storage full.  Thread-local so that concurrent
 // callers (the reference runs interpP2G from an OpenMP team, one thread per
 // species, src/sputniPIC_CPU.cpp:250-256) never clear each other's flags.
 struct StatusBuf {
@@ -196,7 +196,7 @@ int begin_call(hipStream_t s, unsigned int **st) {
   *st = status_buffer(&e);
   if (!*st) return hip_fail(e, "status buffer");
   if (!g_async.load()) {
-    e = spic::zero_u32(*st, 2, s);
+    e = spic::zero_u32(*st, 3, s);
     if (e != hipSuccess) return hip_fail(e, "status reset");
   }
   return SPIC_OK;
@@ -205,13 +205,17 @@ int begin_call(hipStream_t s, unsigned int **st) {
 int read_status(hipStream_t s, unsigned int *st, bool clear) {
   hipError_t e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "kernel execution");
-  unsigned int h[2] = {0, 0};
+  unsigned int h[3] = {0, 0, 0};
   e = hipMemcpy(h, st, sizeof(h), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "status readback");
-  if (clear && (h[0] || h[1])) {
+  if (clear && (h[0] || h[1] || h[2])) {
     e = hipMemset(st, 0, sizeof(h));
     if (e != hipSuccess) return hip_fail(e, "status reset");
   }
+  if (h[2])
+    return fail(SPIC_ENOMEM,
+                "engine particle storage full: %u workgroups could not place "
+                "a particle", h[2]);
   if (h[0] || h[1])
     return fail(SPIC_EDOMAIN,
                 "particles left the grid by more than one cell (%u mover "

@@ -1,17 +1,15 @@
-// spic_engine.cpp -- the tile-resident engine of include/spic.h: particle
-// storage in tile order, the sort schedule, and the one-launch particle
-// phase.  Host code only; the kernels are in spic_tiled.inl and
-// spic_tilesort.hip.
+// spic_engine.cpp -- the tile-resident engine of include/spic.h: binned
+// particle storage, the sort schedule, and the one-launch particle phase.
+// Host code only; the kernels are in spic_binned.inl and spic_binsort.hip.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <vector>
 
 #include "spic.h"
+#include "spic_binned.hpp"
 #include "spic_internal.hpp"
-#include "spic_tiled.hpp"
 
 using spic::fail;
 using spic::hip_fail;
@@ -21,7 +19,8 @@ struct spic_engine {
   void *xn = nullptr, *yn = nullptr, *zn = nullptr;
   int ns = 0;
   int esz = 8;                     // bytes per particle scalar
-  spic::TileLayout L{};
+  spic::BinLayout L{};
+  int slack_pct = 12;
   int sort_every = 0;
   int lds_fp64 = 1;
   int64_t phases = 0, sorts = 0;
@@ -31,13 +30,18 @@ struct spic_engine {
     int64_t n = 0;
     double qom = 0;
     int niter = 3, nsub = 1;
-    void *buf[2][7] = {};          // two sets of x y z u v w q
+    int64_t slack = 0;             // free slots per segment after a sort
+    int64_t cap = 0;               // slots per buffer
+    unsigned int ovf_cap = 0;      // overflow slots (= n: nothing is ever lost)
+    void *buf[2][7] = {};          // two buffers of x y z u v w q
     uint32_t *id[2] = {nullptr, nullptr};
-    int cur = 0;                   // which set holds the particles
-    int64_t *tile_start = nullptr; // [ntiles + 1]
+    unsigned int *cnt[2] = {nullptr, nullptr};  // [ntiles + 1] per buffer
+    int64_t *seg[2] = {nullptr, nullptr};       // [ntiles + 1] per layout
+    int cur = 0;                   // buffer holding the particles
+    int segi = 0;                  // layout in use
   } sp[spic::kMaxSpecies];
-  unsigned long long *counters = nullptr;  // see TiledArgs::counters
-  double *partials = nullptr;      // see TiledArgs::partials
+  unsigned long long *counters = nullptr;  // see BinnedArgs::counters
+  double *partials = nullptr;      // see BinnedArgs::partials
   size_t partials_bytes = 0;
   void *ws = nullptr;              // sort workspace
   size_t ws_bytes = 0;
@@ -45,6 +49,8 @@ struct spic_engine {
 };
 
 namespace {
+
+constexpr int kOverflowBlocks = 256;
 
 int ceil_log2(long v) {
   int b = 0;
@@ -63,75 +69,107 @@ void free_species(spic_engine *e, spic_engine::Species &s) {
     for (int a = 0; a < 7; ++a)
       if (s.buf[b][a]) (void)hipFree(s.buf[b][a]);
     if (s.id[b]) (void)hipFree(s.id[b]);
+    if (s.cnt[b]) (void)hipFree(s.cnt[b]);
+    if (s.seg[b]) (void)hipFree(s.seg[b]);
   }
-  if (s.tile_start) (void)hipFree(s.tile_start);
   s = spic_engine::Species();
   (void)e;
 }
 
+hipError_t ensure_workspace(spic_engine *e, size_t need, hipStream_t s) {
+  if (need <= e->ws_bytes) return hipSuccess;
+  hipError_t err;
+  if (e->ws) {
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) return err;
+    (void)hipFree(e->ws);
+    e->bytes -= e->ws_bytes;
+    e->ws = nullptr;
+    e->ws_bytes = 0;
+  }
+  if ((err = dmalloc(e, &e->ws, need)) != hipSuccess) return err;
+  e->ws_bytes = need;
+  return hipSuccess;
+}
+
+// Sort species `is` into the other buffer and the other layout.  With
+// `src` given, the particles come densely from the caller's arrays instead.
 template <typename T>
-hipError_t sort_species(spic_engine *e, int is, hipStream_t s) {
+hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
   auto &sp = e->sp[is];
   const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
-  T *src[7], *dst[7];
+  spic::BinLayout L = e->L;
+  L.slack = sp.slack;
+  const int from = sp.cur, to = src ? 0 : 1 - sp.cur;
+  const int lay_from = sp.segi, lay_to = src ? 0 : 1 - sp.segi;
+  T *in[7], *out[7];
   for (int a = 0; a < 7; ++a) {
-    src[a] = (T *)sp.buf[sp.cur][a];
-    dst[a] = (T *)sp.buf[1 - sp.cur][a];
+    in[a] = src ? src[a] : (T *)sp.buf[from][a];
+    out[a] = (T *)sp.buf[to][a];
   }
+  const uint32_t *in_id = src ? nullptr : sp.id[from];
+  const int64_t *in_seg = src ? nullptr : sp.seg[lay_from];
+  const unsigned int *in_cnt = src ? nullptr : sp.cnt[from];
   size_t need = 0;
-  hipError_t err = spic::tile_sort<T>(g, e->L, (int)sp.n, src, sp.id[sp.cur], dst,
-                                      sp.id[1 - sp.cur], sp.tile_start, nullptr, 0,
-                                      &need, s);
+  hipError_t err = spic::bin_sort<T>(g, L, sp.n, in, in_id, in_seg, in_cnt, sp.ovf_cap,
+                                     out, sp.id[to], sp.seg[lay_to], sp.cnt[to],
+                                     nullptr, 0, &need, s);
   if (err != hipSuccess) return err;
-  if (need > e->ws_bytes) {
-    if (e->ws) {
-      if ((err = hipStreamSynchronize(s)) != hipSuccess) return err;
-      (void)hipFree(e->ws);
-      e->bytes -= e->ws_bytes;
-      e->ws = nullptr;
-      e->ws_bytes = 0;
-    }
-    if ((err = dmalloc(e, &e->ws, need)) != hipSuccess) return err;
-    e->ws_bytes = need;
-  }
-  err = spic::tile_sort<T>(g, e->L, (int)sp.n, src, sp.id[sp.cur], dst,
-                           sp.id[1 - sp.cur], sp.tile_start, e->ws, e->ws_bytes,
-                           nullptr, s);
+  if ((err = ensure_workspace(e, need, s)) != hipSuccess) return err;
+  err = spic::bin_sort<T>(g, L, sp.n, in, in_id, in_seg, in_cnt, sp.ovf_cap, out,
+                          sp.id[to], sp.seg[lay_to], sp.cnt[to], e->ws, e->ws_bytes,
+                          nullptr, s);
   if (err != hipSuccess) return err;
-  sp.cur = 1 - sp.cur;
+  sp.cur = to;
+  sp.segi = lay_to;
   e->sorts++;
   return hipSuccess;
 }
 
-hipError_t sort_any(spic_engine *e, int is, hipStream_t s) {
-  return e->grd.prec == SPIC_F64 ? sort_species<double>(e, is, s)
-                                 : sort_species<float>(e, is, s);
+hipError_t sort_any(spic_engine *e, int is, void *const *src, hipStream_t s) {
+  return e->grd.prec == SPIC_F64
+             ? sort_species<double>(e, is, (double *const *)src, s)
+             : sort_species<float>(e, is, (float *const *)src, s);
 }
 
 template <typename T>
 hipError_t launch_phase(spic_engine *e, const spic_field *field,
                         const spic_moments *mom, const spic_push *push,
                         unsigned int *status, hipStream_t s) {
-  spic::TiledArgs<T> a;
+  spic::BinnedArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.g = spic::kgrid<T>(&e->grd);
   a.f = spic::kfield<T>(field);
+  a.ncx = e->L.ncx; a.ncy = e->L.ncy; a.ncz = e->L.ncz;
   a.ntx = e->L.ntx; a.nty = e->L.nty; a.ntz = e->L.ntz;
+  a.ntiles = e->L.ntiles;
   a.tx = e->L.tx; a.ty = e->L.ty; a.tz = e->L.tz;
+  a.lds_fp64 = e->lds_fp64;
+  a.n_ovf_blocks = kOverflowBlocks;
   a.status = status;
   a.counters = e->counters;
-  a.lds_fp64 = e->lds_fp64;
   a.partials = e->partials;
+  spic::BinnedReduceArgs r;
+  std::memset(&r, 0, sizeof(r));
+  spic::CountArrays zc;
+  std::memset(&zc, 0, sizeof(zc));
   int k = 0;
+  int idx[spic::kMaxSpecies];
   for (int is = 0; is < e->ns; ++is) {
     auto &sp = e->sp[is];
     if (!sp.loaded || sp.n == 0) continue;
-    auto &t = a.sp[k++];
-    t.x = (T *)sp.buf[sp.cur][0]; t.y = (T *)sp.buf[sp.cur][1];
-    t.z = (T *)sp.buf[sp.cur][2]; t.u = (T *)sp.buf[sp.cur][3];
-    t.v = (T *)sp.buf[sp.cur][4]; t.w = (T *)sp.buf[sp.cur][5];
-    t.q = (const T *)sp.buf[sp.cur][6];
-    t.tile_start = sp.tile_start;
+    idx[k] = is;
+    auto &t = a.sp[k];
+    const int in = sp.cur, out = 1 - sp.cur;
+    for (int c = 0; c < 7; ++c) {
+      t.in[c] = (const T *)sp.buf[in][c];
+      t.out[c] = (T *)sp.buf[out][c];
+    }
+    t.id_in = sp.id[in];
+    t.id_out = sp.id[out];
+    t.seg = sp.seg[sp.segi];
+    t.cnt_in = sp.cnt[in];
+    t.cnt_out = sp.cnt[out];
+    t.ovf_cap = sp.ovf_cap;
     spic_species desc;
     std::memset(&desc, 0, sizeof(desc));
     desc.qom = sp.qom;
@@ -139,15 +177,27 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     desc.n_sub_cycles = sp.nsub;
     t.k = spic::kpush<T>(&desc, push);
     t.m = spic::kmom(&mom[is]);
+    r.cnt[k] = sp.cnt[in];
+    r.mom[k] = t.m;
+    zc.p[k] = sp.cnt[out];
+    ++k;
   }
-  a.ns = k;
   if (k == 0) return hipSuccess;
-  const int nn = e->grd.nxn * e->grd.nyn * e->grd.nzn;
-  hipError_t err = spic::strict_mode()
-                       ? spic::spic_strict::launch_tiled_phase<T>(a, e->L.ntiles, s)
-                       : spic::spic_fast::launch_tiled_phase<T>(a, e->L.ntiles, s);
+  a.ns = r.ns = zc.ns = k;
+  zc.n = e->L.ntiles + 1;
+  hipError_t err = spic::zero_counts(zc, s);
   if (err != hipSuccess) return err;
-  return spic::spic_fast::launch_tile_reduce<T>(a, nn, s);
+  err = spic::strict_mode() ? spic::spic_strict::launch_binned_phase<T>(a, s)
+                            : spic::spic_fast::launch_binned_phase<T>(a, s);
+  if (err != hipSuccess) return err;
+  r.nyn = e->grd.nyn; r.nzn = e->grd.nzn;
+  r.ntx = e->L.ntx; r.nty = e->L.nty; r.ntz = e->L.ntz;
+  r.partials = e->partials;
+  const int nn = e->grd.nxn * e->grd.nyn * e->grd.nzn;
+  err = spic::launch_binned_reduce(r, e->L.tx, e->L.ty, e->L.tz, nn, s);
+  if (err != hipSuccess) return err;
+  for (int j = 0; j < k; ++j) e->sp[idx[j]].cur ^= 1;
+  return hipSuccess;
 }
 
 }  // namespace
@@ -162,7 +212,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (rc) return rc;
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
-  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1;
+  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -171,13 +221,13 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->lds_partials == 1) lds_fp64 = 0;
     else if (opts->lds_partials != 0 && opts->lds_partials != 2)
       return fail(SPIC_EINVAL, "lds_partials must be 0, 1 or 2");
+    if (opts->slack_pct < 0 || opts->slack_pct > 1000)
+      return fail(SPIC_EINVAL, "slack_pct must be 0..1000");
+    if (opts->slack_pct) slack_pct = opts->slack_pct;
   }
-  const bool shape_ok = (tx == 4 && ty == 4 && tz == 4) ||
-                        (tx == 8 && ty == 4 && tz == 4) ||
-                        (tx == 2 && ty == 2 && tz == 2);
+  const bool shape_ok = (tx == 4 && ty == 4 && tz == 4) || (tx == 8 && ty == 4 && tz == 4);
   if (!shape_ok)
-    return fail(SPIC_EINVAL, "tile shape %dx%dx%d not built (4x4x4, 8x4x4, 2x2x2)",
-                tx, ty, tz);
+    return fail(SPIC_EINVAL, "tile shape %dx%dx%d not built (4x4x4, 8x4x4)", tx, ty, tz);
   spic_engine *e = new (std::nothrow) spic_engine();
   if (!e) return fail(SPIC_ENOMEM, "engine allocation");
   e->grd = *grd;
@@ -185,6 +235,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->esz = grd->prec == SPIC_F64 ? 8 : 4;
   e->sort_every = sort_every;
   e->lds_fp64 = lds_fp64;
+  e->slack_pct = slack_pct;
   auto &L = e->L;
   L.ncx = grd->nxc - 2; L.ncy = grd->nyc - 2; L.ncz = grd->nzc - 2;
   L.tx = tx; L.ty = ty; L.tz = tz;
@@ -192,14 +243,15 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   L.nty = (L.ncy + ty - 1) / ty;
   L.ntz = (L.ncz + tz - 1) / tz;
   L.ntiles = L.ntx * L.nty * L.ntz;
-  L.cell_bits = ceil_log2((long)tx * ty * tz);
   L.tile_bits = ceil_log2(L.ntiles);
-  L.rank_bits = 32 - L.cell_bits - L.tile_bits;
-  if (L.rank_bits < 6) {
+  L.cell_bits = ceil_log2((long)tx * ty * tz);
+  L.rank_bits = 32 - L.tile_bits - L.cell_bits;
+  if (L.rank_bits < 4) {
     delete e;
     return fail(SPIC_EUNSUPPORTED, "grid has too many tiles (%d) for 32-bit sort keys",
                 L.ntiles);
   }
+  if (L.rank_bits > 16) L.rank_bits = 16;
   // copy the node tables
   hipError_t err;
   const size_t sx = (size_t)e->esz * grd->nxn, sy = (size_t)e->esz * grd->nyn,
@@ -252,39 +304,49 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
   if (is < 0 || is >= e->ns) return fail(SPIC_EINVAL, "species %d out of range", is);
   int rc = spic::check_species(src, true, true);
   if (rc) return rc;
+  if (src->nop > 0x7FFFFFFFll)
+    return fail(SPIC_EUNSUPPORTED, "engine species hold at most 2^31-1 particles");
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   auto &sp = e->sp[is];
   const int64_t n = src->nop;
+  const int64_t nt = e->L.ntiles;
+  int64_t slack = (n * e->slack_pct + 100 * nt - 1) / (100 * nt);
+  if (slack < 16) slack = 16;
+  // segments (n + nt * slack) then the overflow region (n)
+  const int64_t cap = 2 * n + nt * slack;
   hipError_t err;
-  if (!sp.loaded || sp.n != n) {
+  if (!sp.loaded || sp.cap != cap) {
     if ((err = hipDeviceSynchronize()) != hipSuccess) return hip_fail(err, "engine load");
     free_species(e, sp);
     for (int b = 0; b < 2; ++b) {
       for (int a = 0; a < 7; ++a)
-        if ((err = dmalloc(e, &sp.buf[b][a], (size_t)e->esz * n)) != hipSuccess)
+        if ((err = dmalloc(e, &sp.buf[b][a], (size_t)e->esz * cap)) != hipSuccess)
           return hip_fail(err, "engine particle storage");
-      if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * n)) != hipSuccess)
+      if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
         return hip_fail(err, "engine id storage");
+      if ((err = dmalloc(e, (void **)&sp.cnt[b], sizeof(unsigned int) * (nt + 1))) !=
+              hipSuccess ||
+          (err = dmalloc(e, (void **)&sp.seg[b], sizeof(int64_t) * (nt + 1))) !=
+              hipSuccess)
+        return hip_fail(err, "engine tile tables");
     }
-    if ((err = dmalloc(e, (void **)&sp.tile_start,
-                       sizeof(int64_t) * (e->L.ntiles + 1))) != hipSuccess)
-      return hip_fail(err, "engine tile table");
   }
   sp.n = n;
+  sp.cap = cap;
+  sp.slack = slack;
+  sp.ovf_cap = (unsigned int)(n > 0 ? n : 1);
   sp.qom = src->qom;
   sp.niter = src->NiterMover;
   sp.nsub = src->n_sub_cycles;
   sp.cur = 0;
-  void *srcs[7] = {src->x, src->y, src->z, src->u, src->v, src->w, src->q};
-  for (int a = 0; a < 7; ++a)
-    if ((err = hipMemcpyAsync(sp.buf[0][a], srcs[a], (size_t)e->esz * n,
-                              hipMemcpyDeviceToDevice, s)) != hipSuccess)
-      return hip_fail(err, "engine load copy");
-  if ((err = spic::iota_u32(sp.id[0], (int)n, s)) != hipSuccess)
-    return hip_fail(err, "engine ids");
+  sp.segi = 0;
   sp.loaded = true;
-  if ((err = sort_any(e, is, s)) != hipSuccess) return hip_fail(err, "engine sort");
+  void *srcs[7] = {src->x, src->y, src->z, src->u, src->v, src->w, src->q};
+  if ((err = sort_any(e, is, srcs, s)) != hipSuccess) {
+    sp.loaded = false;
+    return hip_fail(err, "engine sort");
+  }
   if (!spic::async_mode() && (err = hipStreamSynchronize(s)) != hipSuccess)
     return hip_fail(err, "engine load");
   return SPIC_OK;
@@ -303,13 +365,14 @@ int spic_engine_store(spic_engine *e, int is, spic_species *dst, void *stream) {
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   void *dsts[7] = {dst->x, dst->y, dst->z, dst->u, dst->v, dst->w, dst->q};
-  hipError_t err = hipSuccess;
-  for (int a = 0; a < 7 && err == hipSuccess; ++a)
-    err = e->esz == 8
-              ? spic::scatter_by_id_f64((const double *)sp.buf[sp.cur][a],
-                                        sp.id[sp.cur], (double *)dsts[a], (int)sp.n, s)
-              : spic::scatter_by_id_f32((const float *)sp.buf[sp.cur][a],
-                                        sp.id[sp.cur], (float *)dsts[a], (int)sp.n, s);
+  hipError_t err =
+      e->esz == 8
+          ? spic::bin_store<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                    sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
+                                    (double *const *)dsts, s)
+          : spic::bin_store<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                   sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
+                                   (float *const *)dsts, s);
   if (err == hipSuccess && !spic::async_mode()) err = hipStreamSynchronize(s);
   return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine store");
 }
@@ -320,7 +383,7 @@ int spic_engine_sort(spic_engine *e, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   for (int is = 0; is < e->ns; ++is) {
     if (!e->sp[is].loaded) continue;
-    hipError_t err = sort_any(e, is, s);
+    hipError_t err = sort_any(e, is, nullptr, s);
     if (err != hipSuccess) return hip_fail(err, "engine sort");
   }
   if (!spic::async_mode()) {
@@ -347,7 +410,8 @@ int spic_engine_phase(spic_engine *e, const spic_field *field,
   if (e->sort_every > 0 && e->phases > 0 && e->phases % e->sort_every == 0) {
     for (int is = 0; is < e->ns; ++is) {
       if (!e->sp[is].loaded) continue;
-      if ((err = sort_any(e, is, s)) != hipSuccess) return hip_fail(err, "engine sort");
+      if ((err = sort_any(e, is, nullptr, s)) != hipSuccess)
+        return hip_fail(err, "engine sort");
     }
   }
   unsigned int *st;
@@ -360,6 +424,7 @@ int spic_engine_phase(spic_engine *e, const spic_field *field,
 
 int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
   if (!e || !st) return fail(SPIC_EINVAL, "NULL argument");
+  std::memset(st, 0, sizeof(*st));
   st->phases = e->phases;
   st->sorts = e->sorts;
   st->ntiles = e->L.ntiles;
@@ -374,6 +439,15 @@ int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
   st->global_deposits = (int64_t)c[1];
   st->lds_gathers = (int64_t)c[2];
   st->lds_deposits = (int64_t)c[3];
+  for (int is = 0; is < e->ns; ++is) {
+    const auto &sp = e->sp[is];
+    if (!sp.loaded) continue;
+    unsigned int ovf = 0;
+    err = hipMemcpy(&ovf, sp.cnt[sp.cur] + e->L.ntiles, sizeof(ovf),
+                    hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(err, "engine overflow count");
+    st->overflow += ovf;
+  }
   return SPIC_OK;
 }
 

@@ -1,13 +1,14 @@
 """GPU parity of the tile-resident engine (spic_engine_* through the C ABI):
-one launch per particle phase over tile-sorted particles, LDS-staged fields,
-LDS moment accumulation.
+one launch per particle phase over particles binned by tile, LDS-staged
+fields, per-cell LDS buckets and register sums for the deposit, particles
+re-binned into their destination tile's segment as they are written.
 
 Tolerances: in strict arithmetic the particle state after any number of
 phases is bit-identical to the oracle (and so to the reference), whatever
-the tile shape and sort schedule, because sorting only reorders particles
-and the arithmetic per particle is the reference's; node moments then
-differ only in summation order (LDS partials + FP64 global adds): 1e-13 of
-each array's max magnitude in FP64, 1e-5 in FP32 (the FP32-form bound).
+the tile shape, sort schedule or overflow traffic, because binning only
+reorders particles and the arithmetic per particle is the reference's; node
+moments then differ only in summation order: 1e-13 of each array's max
+magnitude in FP64, 1e-5 in FP32 (the FP32-form bound).
 """
 from __future__ import annotations
 
@@ -41,7 +42,8 @@ def oracle_small(prec):
     return fresh
 
 
-def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0):
+def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0,
+               slack_pct=0):
     c = st.cfg
     grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
     dt = np.float64 if prec == "f64" else np.float32
@@ -50,7 +52,7 @@ def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0
     ids = [S.InterpDensSpecies(grd) for _ in range(c.ns)]
     param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
                          PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
-    eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every)
+    eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every, slack_pct=slack_pct)
     for s, p in enumerate(parts):
         eng.load(s, p)
     for _ in range(phases):
@@ -85,7 +87,7 @@ def test_engine_strict_matches_oracle_bitwise(prec):
                          f"s{s} {n}")
 
 
-@pytest.mark.parametrize("tile,sort_every", [((8, 4, 4), 1), ((2, 2, 2), 2), ((4, 4, 4), 1)])
+@pytest.mark.parametrize("tile,sort_every", [((8, 4, 4), 1), ((4, 4, 4), 2), ((4, 4, 4), 1)])
 def test_engine_tile_shapes_and_sorting_do_not_change_trajectories(tile, sort_every):
     st = oracle_small("f64")
     S.set_strict_fp(True)
@@ -108,7 +110,7 @@ def test_engine_fixture_cycles(name, prec):
     data = load_case(name, prec)
     st = oracle_from_fixture(name, prec, data)
     S.set_strict_fp(True)
-    got_p, got_m, _ = run_engine(st, prec, phases=2, tile=(2, 2, 2))
+    got_p, got_m, _ = run_engine(st, prec, phases=2)
     for s in range(st.cfg.ns):
         for k in O.PARTICLE_NAMES:
             assert_bitwise(got_p[s][k], data[f"c2_s{s}_{k}"], f"c2 s{s} {k}")
@@ -117,26 +119,69 @@ def test_engine_fixture_cycles(name, prec):
                          f"c2 s{s} {n}")
 
 
-def test_engine_stale_tiles_take_the_global_path():
+def test_engine_fast_particles_cross_tiles_every_phase():
     """gem_hot particles (E set to zero so speeds stay below one cell per
-    step) cross cells every phase; without re-sorting they leave their
-    tile's halo and must still be pushed and deposited exactly."""
+    step) cross cells and tiles every phase; each phase re-bins them and
+    the trajectories stay exact."""
     data = load_case("gem_hot", "f64")
     st = oracle_from_fixture("gem_hot", "f64", data)
     for n in ("Ex", "Ey", "Ez"):
         st.fields[n][:] = 0.0
     x0 = [p["x"].copy() for p in st.parts]
     S.set_strict_fp(True)
-    got_p, got_m, _ = run_engine(st, "f64", phases=8, tile=(2, 2, 2), sort_every=0)
+    got_p, got_m, _ = run_engine(st, "f64", phases=8, sort_every=0)
     oracle_phases(st, 8)
     dx = st.cfg.Lx / st.cfg.nxc
     moved = sum(int(np.sum(np.abs(p["x"] - x) > 3 * dx)) for p, x in zip(st.parts, x0))
-    assert moved > 100, moved  # many particles are outside their tile's halo
+    assert moved > 100, moved
     for s in range(st.cfg.ns):
         for k in O.PARTICLE_NAMES:
             assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
         for n in O.MOMENT_NAMES:
             assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_overflow_region(prec):
+    """A full segment spills into the overflow region, which the next phase
+    pushes with the global-memory kernel.  gem_hot grid (8x4x4 cells, two
+    4x4x4 tiles along x) with no fields: species 0's particles in the last
+    cell layer of tile 0 drift +0.9 cell per step into tile 1, all others
+    rest, so tile 1 gains 32 particles against 16 free slots."""
+    data = load_case("gem_hot", prec)
+    st = oracle_from_fixture("gem_hot", prec, data)
+    for n in O.FIELD_NAMES:
+        st.fields[n][:] = 0.0
+    c = st.cfg
+    dx = c.Lx / c.nxc
+    for s, p in enumerate(st.parts):
+        for k in ("u", "v", "w"):
+            p[k][:] = 0.0
+        if s == 0:
+            last = (p["x"] >= 3 * dx) & (p["x"] < 4 * dx)
+            p["u"][last] = 0.9 * dx / c.dt
+            assert int(np.sum(last)) == 32
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, prec, phases=3)
+    oracle_phases(st, 3)
+    assert stats["overflow"] > 0, stats
+    assert stats["global_deposits"] > 0, stats
+    for s in range(c.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_load_store_round_trip(prec):
+    st = oracle_small(prec)
+    got_p, _, stats = run_engine(st, prec, phases=0)
+    assert stats["sorts"] == st.cfg.ns and stats["overflow"] == 0
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
 
 
 @pytest.mark.parametrize("prec", PRECS)

@@ -357,3 +357,82 @@ def test_sort_is_stable_permutation_by_bin(sort_cps):
     after = np.stack([p[k] for k in O.PARTICLE_NAMES], axis=1)
     assert_bitwise(after, before[order], "stable bin order")
     assert np.all(np.isin(after[-20:, 0], [-1.0]))
+
+
+# --------------------------------------------------------------------------
+# After the deposit: the ghost-node fold (applyBCids) and the species sum
+# (sumOverSpecies).
+# --------------------------------------------------------------------------
+def _random_moments(st, seed, integers=False):
+    rng = np.random.default_rng(seed)
+    for s in range(st.cfg.ns):
+        for n in O.MOMENT_NAMES:
+            if integers:
+                st.moments[s][n][:] = rng.integers(-1000, 1000, st.nn).astype(st.dt)
+            else:
+                st.moments[s][n][:] = rng.normal(0, 1, st.nn).astype(st.dt)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("per", [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+def test_oracle_fold_matches_live_reference_all_periodicities(prec, per):
+    """The fold of every periodicity combination against the reference's
+    applyBCids on the same random moments: edges and corners are folded
+    once per axis in the reference's order, so any order slip shows."""
+    if not O.ref_available(prec):
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    cfg = O.GemConfig(nxc=6, nyc=5, nzc=4, npcelx=[1] * 4, npcely=[1] * 4,
+                      npcelz=[1] * 4, PERIODICX=bool(per[0]), PERIODICY=bool(per[1]),
+                      PERIODICZ=bool(per[2]))
+    ref = O.RefState(cfg, prec)
+    st = O.State(cfg, prec, build_ic=False)
+    _random_moments(st, seed=sum(per) + 4 * per[0] + 2 * per[1])
+    for s in range(cfg.ns):
+        for w, n in enumerate(O.MOMENT_NAMES):
+            ref.set_moment(s, w, st.moments[s][n].astype(np.float64))
+        ref.apply_bc_ids(s)
+        st.apply_bc_ids(s)
+        for w, n in enumerate(O.MOMENT_NAMES):
+            assert_bitwise(st.moments[s][n], ref.moment(s, w).astype(st.dt), f"s{s} {n}")
+    ref.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_oracle_species_sum_matches_live_reference(prec):
+    if not O.ref_available(prec):
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    cfg = O.GemConfig(nxc=6, nyc=5, nzc=4, npcelx=[1] * 4, npcely=[1] * 4,
+                      npcelz=[1] * 4)
+    ref = O.RefState(cfg, prec)
+    st = O.State(cfg, prec, build_ic=False)
+    _random_moments(st, seed=11)
+    for s in range(cfg.ns):
+        for w, n in enumerate(O.MOMENT_NAMES):
+            ref.set_moment(s, w, st.moments[s][n].astype(np.float64))
+    got, want = st.sum_species(), ref.sum_species()
+    for n in O.MOMENT_NAMES:
+        assert_bitwise(got[n], want[n].astype(st.dt), n)
+    ref.close()
+
+
+def test_species_sum_exact_on_integers_and_identity():
+    """Integer-valued moments sum exactly in any order: the net must equal
+    the exact sum, which catches a species added twice or dropped and a
+    moment or node paired with the wrong one; a net that starts nonzero
+    keeps its value (+=); one nonzero species passes through unchanged."""
+    cfg = O.GemConfig(nxc=6, nyc=5, nzc=4, npcelx=[1] * 4, npcely=[1] * 4,
+                      npcelz=[1] * 4)
+    st = O.State(cfg, "f64", build_ic=False)
+    _random_moments(st, seed=3, integers=True)
+    start = {n: np.arange(st.nn, dtype=np.float64) * (k + 1)
+             for k, n in enumerate(O.MOMENT_NAMES)}
+    net = st.sum_species(start)
+    for k, n in enumerate(O.MOMENT_NAMES):
+        exact = start[n] + sum(st.moments[s][n] for s in range(cfg.ns))
+        assert_bitwise(net[n], exact, n)
+    for s in range(1, cfg.ns):
+        for n in O.MOMENT_NAMES:
+            st.moments[s][n][:] = 0
+    net = st.sum_species()
+    for n in O.MOMENT_NAMES:
+        assert_bitwise(net[n], st.moments[0][n], n)
