@@ -45,7 +45,7 @@ def algorithmic_bytes(n_particles: int, nn: int, w: int, n_species: int) -> int:
 
 
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
-             tile) -> dict:
+             tile, compare: bool = False) -> dict:
     import ctypes as C
 
     import sputnipic_amd as S
@@ -127,9 +127,17 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
                 if timed:
                     marks.append(("kernel", a, b))
 
+    out_moments1 = None
     try:
-        for _ in range(warmup):
+        for w_ in range(warmup):
             cycle(False)
+            if w_ == 0 and compare:
+                # after one cycle from the initial condition, folded
+                stream.synchronize()
+                S.set_async(False)
+                S.applyBCids_all(moms, grd, param)
+                out_moments1 = [m.download() for m in moms]
+                S.set_async(True)
         stream.synchronize()
         check(L.spic_check_domain(stream.ptr), "domain check after warmup")
         state["ev"] = 0
@@ -188,6 +196,23 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         "sort_every": sort_every if eng is not None else None,
         "charge_rel_err": q_err,
     }
+    # state after the last cycle, kept (not printed) for the FP32-against-
+    # FP64 comparison: node moments after the ghost-node fold (applyBCids,
+    # which sums the periodic duplicate nodes, so a particle that wraps in
+    # one precision and not in the other compares equal), and x..w in load
+    # order
+    out_moments = out_parts = None
+    if compare:
+        S.applyBCids_all(moms, grd, param)
+        out_moments = [m.download() for m in moms]
+        if eng is not None:
+            out_parts = []
+            for s in range(NS):
+                dp = S.DeviceParticles(counts[s], 0.0, prec)
+                eng.store(s, dp)
+                out_parts.append({k: dp.d[k].download() for k in ("x", "y", "z", "u", "v", "w")})
+                for d in dp.d.values():
+                    d.free()
     if eng is not None:
         st = eng.stats()
         out["engine_device_bytes"] = st["device_bytes"]
@@ -200,7 +225,147 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         for p in parts:
             for d in p.d.values():
                 d.free()
+    out["_moments"] = out_moments
+    out["_moments1"] = out_moments1
+    out["_parts"] = out_parts
     return out
+
+
+STRESS_CYCLES = 50
+STRESS_SORT_EVERY = 5
+
+
+def run_stress(warmup: int) -> dict:
+    """The combined stress shape: the 128x64x64 Harris sheet with its four
+    species at 5x5x5 per cell plus four uniform beams (electrons and ions,
+    x drift +-0.1, thermal speed 0.02) at 1x5x5 = 25 per cell, 314,572,800
+    particles in FP32; the engine rebuilds its tile segments every 5 phases;
+    50 cycles (zero the moments, engine phase) are captured in one HIP graph
+    and timed as one launch."""
+    import ctypes as C
+
+    import sputnipic_amd as S
+    from sputnipic_amd._capi import check, lib
+
+    L = lib()
+    grd = S.Grid(*GRID, *BOX, "f32")
+    gem = S.GemParams()
+    field = S.gem_fields(grd, gem)
+    species = S.harris_species(PPC) + S.beam_species()
+    ns = len(species)
+    eng = S.Engine(grd, ns, sort_every=STRESS_SORT_EVERY)
+    stream = S.Stream()
+    n_total = 0
+    for s, sp in enumerate(species):
+        p = S.gem_particles(grd, gem, sp, s, seed=0)
+        n_total += p.npmax
+        eng.load(s, p, stream)
+        stream.synchronize()
+        for d in p.d.values():
+            d.free()
+    log(f"stress: {n_total} particles in {ns} species, FP32, segments rebuilt every "
+        f"{STRESS_SORT_EVERY} phases")
+    moms = [S.InterpDensSpecies(grd) for _ in range(ns)]
+    param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
+
+    def cycle():
+        S.setZeroSpeciesDensities(moms, grd, ns, stream)
+        eng.phase(field, moms, param, stream)
+
+    S.set_async(True)
+    try:
+        for _ in range(warmup):
+            cycle()
+        stream.synchronize()
+        check(L.spic_check_domain(stream.ptr), "domain check after warmup")
+        S.set_async(True)
+        t0 = time.perf_counter()
+        with S.Graph(stream) as graph:
+            for _ in range(STRESS_CYCLES):
+                cycle()
+        t_capture = time.perf_counter() - t0
+        a, b = S.Event(), S.Event()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        a.record(stream)
+        graph.launch()
+        b.record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        check(L.spic_check_domain(stream.ptr), "domain check")
+    finally:
+        S.set_async(False)
+    gpu_ms = a.elapsed_ms(b)
+    st = eng.stats()
+    graph.close()
+    eng.close()
+    value = n_total * STRESS_CYCLES / wall
+    per_cycle_bytes = algorithmic_bytes(n_total, grd.nn, 4, ns)
+    achieved = per_cycle_bytes * STRESS_CYCLES / (gpu_ms * 1e-3) / 1e9
+    log(f"stress: {value / 1e9:.3f} G particles/s, {1e3 * wall / STRESS_CYCLES:.3f} "
+        f"ms/cycle ({gpu_ms / STRESS_CYCLES:.3f} ms/cycle on the GPU clock), "
+        f"{achieved:.1f} GB/s, capture {t_capture:.2f} s, stats {st}")
+    return {
+        "value": value,
+        "ms_per_step": 1e3 * wall / STRESS_CYCLES,
+        "particles": n_total,
+        "gpu_ms_per_cycle": gpu_ms / STRESS_CYCLES,
+        "achieved_GBs": achieved,
+        "roofline_frac": achieved / HBM_PEAK_GBS,
+        "bytes_per_cycle": per_cycle_bytes,
+        "engine_stats": {k: st[k] for k in ("phases", "sorts", "global_deposits",
+                                            "overflow")},
+    }
+
+
+def form_errors(m32, m64, interior: bool = False) -> dict:
+    """FP32 form against FP64 form after the same cycles from the same
+    initial condition: per moment, max |m32 - m64| over all species and
+    nodes, relative to the largest |m64| of that moment.  With interior,
+    the three node planes next to each reflecting y wall are left out: a
+    particle within one rounding step of the wall can reflect in one
+    precision only, which flips its v and so the sign of its share of Jy,
+    pxy and pyz at those nodes."""
+    import numpy as np
+
+    nxn, nyn, nzn = GRID[0] + 3, GRID[1] + 3, GRID[2] + 3
+    keep = np.ones((nxn, nyn, nzn), bool)
+    if interior:
+        keep[:, :3, :] = False
+        keep[:, nyn - 3:, :] = False
+    keep = keep.reshape(-1)
+    err = {}
+    for n in m64[0]:
+        num = max(float(abs(a[n] - b[n])[keep].max()) for a, b in zip(m32, m64))
+        den = max(float(abs(b[n]).max()) for b in m64)
+        err[n] = num / den if den > 0 else num
+    return err
+
+
+def particle_errors(p32, p64) -> dict:
+    """Per particle component, FP32 form against FP64 form: positions as
+    |dx| over the box length (x and z modulo the periodic length), velocities
+    as |du| over the largest |u| of the FP64 form; the maximum and the root
+    mean square over all particles.  A particle within rounding of a
+    reflecting wall can reflect in one precision only, which flips its
+    velocity component: the maximum shows such particles, the RMS does not."""
+    import numpy as np
+
+    err = {}
+    scale = {"x": BOX[0], "y": BOX[1], "z": BOX[2]}
+    for k in ("u", "v", "w"):
+        scale[k] = max(float(np.abs(b[k]).max()) for b in p64)
+    for k in ("x", "y", "z", "u", "v", "w"):
+        worst, sq, cnt = 0.0, 0.0, 0
+        for a, b in zip(p32, p64):
+            d = np.abs(a[k].astype(np.float64) - b[k])
+            if k in ("x", "z"):
+                d = np.minimum(d, scale[k] - d)
+            worst = max(worst, float(d.max()))
+            sq += float(np.dot(d, d))
+            cnt += d.size
+        err[k] = {"max": worst / scale[k], "rms": (sq / max(cnt, 1)) ** 0.5 / scale[k]}
+    return err
 
 
 class _StdoutToStderr:
@@ -275,6 +440,9 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
+    ap.add_argument("--workload", choices=("gem", "stress"), default="gem",
+                    help="gem: the 262M-particle Harris sheet (default); stress: 8 "
+                         "species, 314.6M particles, FP32, 50 cycles in one HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once device to device (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -300,11 +468,73 @@ def main() -> int:
     if S.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if args.workload == "stress":
+        f = run_stress(args.warmup)
+        line = {
+            "metric": METRIC,
+            "value": f["value"],
+            "unit": "particles/s",
+            "n_gpus": 1,
+            "steps": STRESS_CYCLES,
+            "warmup": args.warmup,
+            "ms_per_step": f["ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {
+                "workload": "stress: Harris-sheet GEM 3D, 128x64x64 cells, 4 Harris species "
+                            "at 5x5x5 per cell + 4 x-beams (+-0.1 drift, uth 0.02) at 1x5x5 "
+                            "per cell = 314,572,800 particles, FP32, tile segments rebuilt "
+                            "every 5 phases, 50 cycles captured in one HIP graph",
+                "particles": f["particles"],
+            },
+            "roofline": {
+                "bound": "hbm",
+                "achieved": f["achieved_GBs"],
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": f["roofline_frac"],
+                "traffic": None,
+                "kernel": "whole captured cycle (zero, phase, overflow, reduce, sorts)",
+                "algorithmic_bytes_per_launch": f["bytes_per_cycle"],
+            },
+            "stress": f,
+            "cpu_baseline": None,
+        }
+        print(json.dumps(line), flush=True)
+        return 0
     forms = {}
     for prec in args.forms.split(","):
         tile = tuple(int(t) for t in args.tile.split("x"))
         forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, args.sort_every,
-                               tile)
+                               tile, compare=len(args.forms.split(",")) > 1)
+    errors = errors1 = errorsi = errors1i = perrors = None
+    if "f64" in forms and "f32" in forms:
+        if forms["f64"]["_moments1"] is not None and forms["f32"]["_moments1"] is not None:
+            errors1 = form_errors(forms["f32"]["_moments1"], forms["f64"]["_moments1"])
+            errors1i = form_errors(forms["f32"]["_moments1"], forms["f64"]["_moments1"],
+                                   interior=True)
+            log("FP32 form against FP64 form after one cycle, max relative moment error: " +
+                ", ".join(f"{k} {v:.1e}" for k, v in errors1.items()))
+            log("  the same away from the reflecting walls: " +
+                ", ".join(f"{k} {v:.1e}" for k, v in errors1i.items()))
+        errors = form_errors(forms["f32"]["_moments"], forms["f64"]["_moments"])
+        errorsi = form_errors(forms["f32"]["_moments"], forms["f64"]["_moments"],
+                              interior=True)
+        log("FP32 form against FP64 form, max relative moment error: " +
+            ", ".join(f"{k} {v:.1e}" for k, v in errors.items()))
+        log("  the same away from the reflecting walls: " +
+            ", ".join(f"{k} {v:.1e}" for k, v in errorsi.items()))
+        if forms["f64"]["_parts"] is not None and forms["f32"]["_parts"] is not None:
+            perrors = particle_errors(forms["f32"]["_parts"], forms["f64"]["_parts"])
+            log("FP32 form against FP64 form, particle error (max, rms): " +
+                ", ".join(f"{k} {v['max']:.1e} {v['rms']:.1e}" for k, v in perrors.items()))
+    for f_ in forms.values():
+        f_.pop("_moments", None)
+        f_.pop("_moments1", None)
+        f_.pop("_parts", None)
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
     cpu = None
@@ -348,6 +578,22 @@ def main() -> int:
     }
     if "f64" in forms and "f32" in forms:
         line["fp32_over_fp64"] = forms["f32"]["value"] / forms["f64"]["value"]
+        line["fp32_error"] = {
+            "cycles": args.warmup + args.steps,
+            "moments_max_rel_err_after_1_cycle": errors1,
+            "moments_max_rel_err_after_1_cycle_interior": errors1i,
+            "moments_max_rel_err": errors,
+            "moments_max_rel_err_interior": errorsi,
+            "particles_rel_err": perrors,
+            "definition": "FP32 form against FP64 form after the same cycles from the same "
+                          "initial condition (FP32 values rounded from the FP64 "
+                          "generator's), frozen fields.  Moments: after the ghost-node fold "
+                          "(applyBCids), per moment max |FP32 - FP64| over species and nodes "
+                          "/ max |FP64|; interior: without the 3 node planes next to each reflecting "
+                          "y wall, where a particle within rounding of the wall reflects in one "
+                          "precision only.  Particles: max and rms over all particles of |dx| / box "
+                          "length (x, z modulo the period) and |du| / max |u|",
+        }
     print(json.dumps(line), flush=True)
     return 0
 

@@ -300,6 +300,21 @@ int spic_stream_create(void **stream);
 int spic_stream_destroy(void *stream);
 int spic_stream_synchronize(void *stream);
 
+/* HIP graphs: capture a run of calls on a created stream (in async mode:
+ * spic_set_async(1)) once and replay it with one launch.  Every entry point
+ * above that takes a stream can be captured once its buffers exist (an
+ * engine after spic_engine_load, the moment arrays after allocation); none
+ * allocates or synchronizes in async mode.  A replay repeats the captured
+ * work on the same device buffers: an engine phase captures which of its
+ * two particle buffers it reads, so capture an even number of phases and
+ * sorts to replay a run more than once.  Replays do not advance the host
+ * counters of spic_engine_get_stats. */
+typedef struct spic_graph spic_graph;
+int spic_graph_begin(void *stream);
+int spic_graph_end(void *stream, spic_graph **graph);
+int spic_graph_launch(spic_graph *graph, void *stream);
+int spic_graph_destroy(spic_graph *graph);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

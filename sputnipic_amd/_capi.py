@@ -129,6 +129,10 @@ SIGNATURES = {
     "spic_event_destroy": (C.c_int, [_P]),
     "spic_event_record": (C.c_int, [_P, _P]),
     "spic_event_elapsed_ms": (C.c_int, [C.POINTER(C.c_float), _P, _P]),
+    "spic_graph_begin": (C.c_int, [_P]),
+    "spic_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "spic_graph_launch": (C.c_int, [_P, _P]),
+    "spic_graph_destroy": (C.c_int, [_P]),
     "spic_last_error": (C.c_char_p, []),
     "spic_abi_version": (C.c_int, []),
     "spic_set_async": (C.c_int, [C.c_int]),

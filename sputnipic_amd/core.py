@@ -232,6 +232,12 @@ def set_async(async_mode: bool) -> None:
     check(lib().spic_set_async(int(bool(async_mode))), "spic_set_async")
 
 
+def check_domain(stream: Optional["Stream"] = None) -> None:
+    """Wait for the stream and raise if any call since the last check found
+    particles outside the grid (the status an async-mode call leaves)."""
+    check(lib().spic_check_domain(stream.ptr if stream else None), "spic_check_domain")
+
+
 def mover_PC(part: Particles, field: EMField, grd: Grid, param: Parameters) -> int:
     """mover_PC (src/Mover.cpp:4-162); returns 0 like the reference."""
     _check_prec(part, grd)
@@ -267,12 +273,14 @@ def particle_sort(param: Parameters, part: Particles, grd: Grid) -> None:
           "particle_sort")
 
 
-def setZeroSpeciesDensities(ids: Sequence[InterpDensSpecies], grd: Grid, ns: int) -> None:
+def setZeroSpeciesDensities(ids: Sequence[InterpDensSpecies], grd: Grid, ns: int,
+                            stream: Optional["Stream"] = None) -> None:
     """setZeroSpeciesDensities (src/InterpDensNet.cpp:40-75), node moments."""
     g = grd.c_struct()
     for is_ in range(ns):
         m = ids[is_].c_struct()
-        check(lib().spic_zero_moments(C.byref(g), C.byref(m), None), "zero moments")
+        check(lib().spic_zero_moments(C.byref(g), C.byref(m), stream.ptr if stream else None),
+              "zero moments")
 
 
 def applyBCids(ids: InterpDensSpecies, grd: Grid, param: Parameters) -> None:
@@ -344,6 +352,8 @@ class GemSpecies:
     sheet: bool = True
 
     def count(self, grd: Grid) -> int:
+        # the reference's lattice loop runs its z index to npcely
+        # (src/IC.cpp:128), so a cell holds npcelx * npcely * npcely
         cells = (grd.nxc - 2) * (grd.nyc - 2) * (grd.nzc - 2)
         return cells * self.npcelx * self.npcely * self.npcely
 
@@ -356,6 +366,19 @@ def harris_species(ppc=(3, 3, 3)):
         GemSpecies(1.0, 1.0, px, py, pz, 0.0126, 0.0126, 0.0126, 0, 0, -0.0325, True),
         GemSpecies(-64.0, 0.1, px, py, pz, 0.045, 0.045, 0.045, 0, 0, 0.0, False),
         GemSpecies(1.0, 0.1, px, py, pz, 0.0126, 0.0126, 0.0126, 0, 0, 0.0, False),
+    ]
+
+
+def beam_species(ppc=(1, 5, 5), drift: float = 0.1, uth: float = 0.02,
+                 rho: float = 0.1):
+    """Four uniform beams along x: counter-streaming electrons (qom -64,
+    u0 = +drift / -drift) and ions (qom 1, likewise), thermal speed uth."""
+    px, py, pz = ppc
+    return [
+        GemSpecies(-64.0, rho, px, py, pz, uth, uth, uth, +drift, 0, 0, False),
+        GemSpecies(-64.0, rho, px, py, pz, uth, uth, uth, -drift, 0, 0, False),
+        GemSpecies(1.0, rho, px, py, pz, uth, uth, uth, +drift, 0, 0, False),
+        GemSpecies(1.0, rho, px, py, pz, uth, uth, uth, -drift, 0, 0, False),
     ]
 
 
@@ -416,6 +439,43 @@ class Event:
         ms = C.c_float()
         check(lib().spic_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr), "event elapsed")
         return ms.value
+
+
+class Graph:
+    """A captured run of calls on one stream (spic_graph_*), replayed with
+    one launch.  Use as ``with Graph(stream) as g: ...`` (capture; async mode
+    is switched on for the capture), then ``g.launch()``."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.ptr = None
+
+    def __enter__(self):
+        set_async(True)
+        check(lib().spic_graph_begin(self.stream.ptr), "spic_graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        g = C.c_void_p()
+        rc = lib().spic_graph_end(self.stream.ptr, C.byref(g))
+        if exc_type is None:
+            check(rc, "spic_graph_end")
+            self.ptr = g.value
+        return False
+
+    def launch(self) -> None:
+        check(lib().spic_graph_launch(self.ptr, self.stream.ptr), "spic_graph_launch")
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().spic_graph_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Engine:

@@ -234,13 +234,14 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
   }
 }
 
-// Tile of the deposit cell (ix, iy, iz), clamped to the physical cells.
-template <typename T>
+// Tile of the deposit cell (ix, iy, iz), clamped to the physical cells;
+// the tile shape is a compile-time constant (divisions become shifts).
+template <int TX, int TY, int TZ, typename T>
 __device__ __forceinline__ int tile_of(const BinnedArgs<T> &a, int ix, int iy, int iz) {
   const int pcx = min(max(ix - 2, 0), a.ncx - 1);
   const int pcy = min(max(iy - 2, 0), a.ncy - 1);
   const int pcz = min(max(iz - 2, 0), a.ncz - 1);
-  return ((pcx / a.tx) * a.nty + pcy / a.ty) * a.ntz + pcz / a.tz;
+  return ((pcx / TX) * a.nty + pcy / TY) * a.ntz + pcz / TZ;
 }
 
 // Output slot of a particle to which its destination tile's counter gave
@@ -354,10 +355,13 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
   return in_block;
 }
 
-// A: type of the LDS moment block and register sums (T, or double for
-// FP32 particles).
+// A: type of the LDS moment block (T, or double for FP32 particles); the
+// per-thread register sums are in the particle precision T.  FP32 kernels
+// ask for three waves per SIMD (at most 168 VGPRs).
 template <typename T, typename A, int TX, int TY, int TZ>
-__global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T> a) {
+__global__ __launch_bounds__(4 * TX * TY * TZ)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+void k_binned_phase(BinnedArgs<T> a) {
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
   constexpr int NT = 4 * NC;        // threads: four per cell
@@ -427,9 +431,9 @@ __global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T>
     for (int i = tid; i < 2 * NC; i += NT) (&s_cnt[0][0])[i] = 0u;
     if (tid < 2) s_nl[tid] = 0u;
     __syncthreads();
-    A acc[20];
+    T acc[20];  // register sums, particle precision
 #pragma unroll
-    for (int k = 0; k < 20; ++k) acc[k] = A(0);
+    for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
     const int64_t nbatches = (nin + NT - 1) / NT;
     for (int64_t j = 0; j < nbatches; ++j) {
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T>
         bad_dep |= bclamp(ix, a.g.nxn);
         bad_dep |= bclamp(iy, a.g.nyn);
         bad_dep |= bclamp(iz, a.g.nzn);
-        dest = tile_of(a, ix, iy, iz);
+        dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
       }
 
       // re-bin: one counter atomic per wave for the particles that stay
@@ -540,15 +544,14 @@ __global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T>
             // the reference's terms: (p_m * (q xi eta zeta iv)) * iv
             const T w0 = pq * xa * yb * (pz - Z0) * iv;  // corner (a, b, 0)
             const T w1 = pq * xa * yb * (Z1 - pz) * iv;  // corner (a, b, 1)
-            add_terms<T, A>(acc, w0, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
-            add_terms<T, A>(acc + 10, w1, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
+            add_terms<T, T>(acc, w0, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
+            add_terms<T, T>(acc + 10, w1, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
           } else {
             // sum of p_m * (q xi eta zeta); the factor iv^2 is applied once
             // when the sums are added to the LDS block
             const T qxy = pq * xa * yb;
-            const A w0 = (A)(qxy * (pz - Z0)), w1 = (A)(qxy * (Z1 - pz));
-            const A pm[10] = {A(1), (A)pu, (A)pv, (A)pw, (A)uu,
-                              (A)uv, (A)uw, (A)vv, (A)vw, (A)ww};
+            const T w0 = qxy * (pz - Z0), w1 = qxy * (Z1 - pz);
+            const T pm[10] = {T(1), pu, pv, pw, uu, uv, uw, vv, vw, ww};
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
               acc[m] += pm[m] * w0;
@@ -592,8 +595,8 @@ __global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T>
       const int nb = ((mlx + 2 - my_a) * BY + (mly + 2 - my_b)) * BZ + (mlz + 2 - cc);
 #pragma unroll
       for (int m = 0; m < 10; ++m)
-        if (acc[cc * 10 + m] != A(0))
-          atomicAdd(&s_mom[m * NB + nb], acc[cc * 10 + m] * scale);
+        if (acc[cc * 10 + m] != T(0))
+          atomicAdd(&s_mom[m * NB + nb], (A)acc[cc * 10 + m] * scale);
     }
     __syncthreads();
     double *dst = a.partials + ((size_t)tile * a.ns + s) * (10 * NB);
@@ -613,7 +616,7 @@ __global__ __launch_bounds__(4 * TX * TY * TZ) void k_binned_phase(BinnedArgs<T>
 
 // The particles of every species' input overflow region: global gathers
 // and deposits, re-binned like the tile kernel's.
-template <typename T>
+template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
   Block<T, 1, 1, 1> none;
   none.fld = nullptr;
@@ -640,7 +643,7 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
       bad_dep |= bclamp(ix, a.g.nxn);
       bad_dep |= bclamp(iy, a.g.nyn);
       bad_dep |= bclamp(iz, a.g.nzn);
-      const int dest = tile_of(a, ix, iy, iz);
+      const int dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
       const int64_t o = out_slot(a, sp, dest, atomicAdd(sp.cnt_out + dest, 1u));
       if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
       else bad_store = true;
@@ -659,25 +662,27 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
 
 template <typename T, typename A>
 hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
-  if (a.tx == 4 && a.ty == 4 && a.tz == 4)
+  if (a.tx == 4 && a.ty == 4 && a.tz == 4) {
     hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4>), dim3(a.ntiles), dim3(256), 0,
                        s, a);
-  else if (a.tx == 8 && a.ty == 4 && a.tz == 4)
+    hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4>), dim3(a.n_ovf_blocks), dim3(256),
+                       0, s, a);
+  } else if (a.tx == 8 && a.ty == 4 && a.tz == 4) {
     hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4>), dim3(a.ntiles), dim3(512), 0,
                        s, a);
-  else
+    hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4>), dim3(a.n_ovf_blocks), dim3(256),
+                       0, s, a);
+  } else {
     return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
   if (a.ntiles <= 0 || a.ns <= 0) return hipSuccess;
-  hipError_t e = (sizeof(T) == 4 && a.lds_fp64) ? launch_binned_shape<T, double>(a, s)
-                                                : launch_binned_shape<T, T>(a, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_binned_overflow<T>, dim3(a.n_ovf_blocks), dim3(256), 0, s, a);
-  return hipGetLastError();
+  return (sizeof(T) == 4 && a.lds_fp64) ? launch_binned_shape<T, double>(a, s)
+                                        : launch_binned_shape<T, T>(a, s);
 }
 
 template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);

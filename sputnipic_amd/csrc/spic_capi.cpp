@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "spic.h"
@@ -456,6 +457,57 @@ int spic_stream_destroy(void *stream) {
 int spic_stream_synchronize(void *stream) {
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+}  // extern "C"
+
+struct spic_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int spic_graph_begin(void *stream) {
+  if (!stream) return fail(SPIC_EINVAL, "graph capture needs a created stream, not NULL");
+  if (!g_async.load())
+    return fail(SPIC_EINVAL, "graph capture needs async mode (spic_set_async(1))");
+  // the per-thread status words must exist before capture: no allocation
+  // may happen while the stream is captured
+  hipError_t e;
+  if (!status_buffer(&e)) return hip_fail(e, "status buffer");
+  e = hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipStreamBeginCapture");
+}
+
+int spic_graph_end(void *stream, spic_graph **out) {
+  if (!out) return fail(SPIC_EINVAL, "graph pointer is NULL");
+  *out = nullptr;
+  spic_graph *g = new (std::nothrow) spic_graph();
+  if (!g) return fail(SPIC_ENOMEM, "graph allocation");
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    spic_graph_destroy(g);
+    return hip_fail(e, "graph capture");
+  }
+  *out = g;
+  return SPIC_OK;
+}
+
+int spic_graph_launch(spic_graph *g, void *stream) {
+  if (!g || !g->exec) return fail(SPIC_EINVAL, "graph is NULL");
+  hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  if (e == hipSuccess && !g_async.load()) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int spic_graph_destroy(spic_graph *g) {
+  if (!g) return SPIC_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return SPIC_OK;
 }
 
 }  // extern "C"
