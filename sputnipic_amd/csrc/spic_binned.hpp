@@ -22,7 +22,7 @@ namespace spic {
 
 constexpr int kMaxSpecies = 8;
 constexpr int kBucketCap = 7;  // LDS bucket slots per cell and batch
-constexpr int kListCap = 64;   // LDS list entries per batch
+constexpr int kListCap = 128;  // LDS list ring entries
 
 template <typename T>
 struct BinnedSpecies {

@@ -370,14 +370,15 @@ void k_binned_phase(BinnedArgs<T> a) {
   // bucket stride per cell: odd, so that the 16 cells a wave reads in the
   // bucket step fall on distinct LDS banks
   constexpr int CS = CAP | 1;
-  constexpr int NL = kListCap;
+  constexpr int NL = kListCap;  // list ring entries
   __shared__ T s_fld[NB * 6];
   __shared__ A s_mom[10 * NB];
   __shared__ T s_xn[BX], s_yn[BY], s_zn[BZ];
   __shared__ T s_bkt[7][NC * CS];  // x y z q u v w of the bucketed particles
   __shared__ T s_lst[7][NL];       // x y z q u v w of the listed particles
+  __shared__ unsigned int s_lcell[NL];  // their deposit cells, packed
   __shared__ unsigned int s_cnt[2][NC];
-  __shared__ unsigned int s_nl[2];
+  __shared__ unsigned int s_nl;    // entries appended to the ring this species
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -429,7 +430,10 @@ void k_binned_phase(BinnedArgs<T> a) {
     if (nin == 0) continue;  // uniform across the workgroup
     for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
     for (int i = tid; i < 2 * NC; i += NT) (&s_cnt[0][0])[i] = 0u;
-    if (tid < 2) s_nl[tid] = 0u;
+    if (tid == 0) s_nl = 0u;
+    // entries [0, lst_done) of the ring have been deposited; the same value
+    // in every thread
+    unsigned int lst_done = 0u;
     __syncthreads();
     T acc[20];  // register sums, particle precision
 #pragma unroll
@@ -502,10 +506,12 @@ void k_binned_phase(BinnedArgs<T> a) {
         }
         if (!done) {
           // halo cell, outside the block or full bucket: to the list
-          const unsigned int k = atomicAdd(&s_nl[buf], 1u);
-          if (k < (unsigned int)NL) {
-            s_lst[0][k] = x; s_lst[1][k] = y; s_lst[2][k] = z; s_lst[3][k] = q;
-            s_lst[4][k] = u; s_lst[5][k] = v; s_lst[6][k] = w;
+          const unsigned int k = atomicAdd(&s_nl, 1u);
+          if (k - lst_done < (unsigned int)NL) {
+            const unsigned int e = k % NL;
+            s_lst[0][e] = x; s_lst[1][e] = y; s_lst[2][e] = z; s_lst[3][e] = q;
+            s_lst[4][e] = u; s_lst[5][e] = v; s_lst[6][e] = w;
+            s_lcell[e] = ((unsigned int)ix << 20) | ((unsigned int)iy << 10) | (unsigned int)iz;
             done = true;
           }
         }
@@ -560,29 +566,32 @@ void k_binned_phase(BinnedArgs<T> a) {
           }
         }
       }
-      // list step: eight threads per listed particle, one node each
+      // list step: once the ring is half full (and after the species' last
+      // batch), all threads deposit its pending entries, eight threads per
+      // entry, one node each.  s_nl is not written again before the next
+      // barrier, so every thread reads the same value and takes the same
+      // decision.
       {
-        const unsigned int nl = min(s_nl[buf], (unsigned int)NL);
-        const int nd = tid & 7;
-        for (unsigned int k = (unsigned int)(tid >> 3); k < nl; k += NT / 8) {
-          const T px = s_lst[0][k], py = s_lst[1][k], pz = s_lst[2][k];
-          const T pq = s_lst[3][k], pu = s_lst[4][k], pv = s_lst[5][k], pw = s_lst[6][k];
-          // the inserting thread's deposit cell, recomputed
-          int jx = bcell_floor<T>(px, a.g.xStart, a.g.invdx, a.g.nxn);
-          int jy = bcell_floor<T>(py, a.g.yStart, a.g.invdy, a.g.nyn);
-          int jz = bcell_floor<T>(pz, a.g.zStart, a.g.invdz, a.g.nzn);
-          bclamp(jx, a.g.nxn);
-          bclamp(jy, a.g.nyn);
-          bclamp(jz, a.g.nzn);
-          const bool lds = deposit_node<T, A, BX, BY, BZ>(
-              a.g, sp.m, s_mom, s_xn, s_yn, s_zn, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz,
-              nd >> 2, (nd >> 1) & 1, nd & 1, px, py, pz, pu, pv, pw, pq);
-          if (nd == 0) {
-            if (lds) ++n_ld;
-            else ++n_gd;
+        const unsigned int nl = s_nl;
+        const unsigned int pend = min(nl - lst_done, (unsigned int)NL);
+        if (pend >= (unsigned int)(NL / 2) || (j == nbatches - 1 && pend > 0u)) {
+          const int nd = tid & 7;
+          for (unsigned int k = (unsigned int)(tid >> 3); k < pend; k += NT / 8) {
+            const unsigned int e = (lst_done + k) % NL;
+            const T px = s_lst[0][e], py = s_lst[1][e], pz = s_lst[2][e];
+            const T pq = s_lst[3][e], pu = s_lst[4][e], pv = s_lst[5][e], pw = s_lst[6][e];
+            const unsigned int c = s_lcell[e];
+            const int jx = (int)(c >> 20), jy = (int)((c >> 10) & 1023u), jz = (int)(c & 1023u);
+            const bool lds = deposit_node<T, A, BX, BY, BZ>(
+                a.g, sp.m, s_mom, s_xn, s_yn, s_zn, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz,
+                nd >> 2, (nd >> 1) & 1, nd & 1, px, py, pz, pu, pv, pw, pq);
+            if (nd == 0) {
+              if (lds) ++n_ld;
+              else ++n_gd;
+            }
           }
+          lst_done = nl;
         }
-        if (tid == 0) s_nl[buf ^ 1] = 0u;  // for the next batch
       }
       __syncthreads();
     }

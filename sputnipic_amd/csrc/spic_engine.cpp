@@ -225,6 +225,11 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
       return fail(SPIC_EINVAL, "slack_pct must be 0..1000");
     if (opts->slack_pct) slack_pct = opts->slack_pct;
   }
+  // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
+  if (grd->nxn > 4096 || grd->nyn > 1024 || grd->nzn > 1024)
+    return fail(SPIC_EUNSUPPORTED,
+                "engine grids hold at most 4096 x 1024 x 1024 nodes (got %d x %d x %d)",
+                grd->nxn, grd->nyn, grd->nzn);
   const bool shape_ok = (tx == 4 && ty == 4 && tz == 4) || (tx == 8 && ty == 4 && tz == 4);
   if (!shape_ok)
     return fail(SPIC_EINVAL, "tile shape %dx%dx%d not built (4x4x4, 8x4x4)", tx, ty, tz);
