@@ -44,6 +44,19 @@ def algorithmic_bytes(n_particles: int, nn: int, w: int, n_species: int) -> int:
     return n_particles * 13 * w + 6 * nn * w + 10 * n_species * nn * 8
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_binned_phase_gem128x64x64_262M_traffic.json")
+
+
+def measured_traffic(prec: str):
+    """HBM bytes per phase launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this workload (scaled by the copy calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return float(json.load(f)["bytes_per_launch"][prec])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
              tile, compare: bool = False) -> dict:
     import ctypes as C
@@ -568,7 +581,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": f["roofline_frac"],
-            "traffic": None,
+            "traffic": measured_traffic(primary) if args.engine == "tiled" else None,
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "kernel": f["kernel"],
             "avg_kernel_ms": f["avg_kernel_ms"],
             "algorithmic_bytes_per_launch": f["bytes_per_launch"],
