@@ -2,15 +2,20 @@
 // per-node reduction and the binned sort / store.  Internal to libspic.so.
 //
 // Binned particle storage of one species (one per buffer of a pair):
-//   slots [seg[t], seg[t+1])      tile t's segment: its particles first,
-//                                  then free slots (slack);
+//   slots [seg[t], seg[t+1])      tile t's segment;
 //   slots [seg[ntiles], +ovf_cap)  overflow region, for particles whose
-//                                  destination segment was full;
-//   cnt[t], t < ntiles             particles written to tile t (may exceed
-//                                  the segment's capacity; the excess went
-//                                  to the overflow region);
-//   cnt[ntiles]                    particles in the overflow region.
-// Both buffers of a pair share seg; the phase reads one and writes the other.
+//                                  destination segment was full.
+// Per buffer, one array of 3 * (ntiles + 1) counters:
+//   cnt[t]                 slots of range t taken (t == ntiles: the overflow
+//                          region); may exceed the range's capacity, the
+//                          excess went to the overflow region;
+//   cnt[NT1 + t], cnt[2 * NT1 + t]   (NT1 = ntiles + 1) the start and length
+//                          of one unused run inside the taken slots: the
+//                          rest of the block a tile's workgroup reserved for
+//                          the particles that stayed in it.
+// Range t holds min(cnt[t], capacity) - length particles, at the taken slots
+// outside the unused run.  Both buffers of a pair share seg; a phase reads
+// one and writes the other.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,15 +29,41 @@ constexpr int kMaxSpecies = 8;
 constexpr int kBucketCap = 7;  // LDS bucket slots per cell and batch
 constexpr int kListCap = 128;  // LDS list ring entries
 
+// The particles of range t (a tile's segment, or t == ntiles: the overflow
+// region) as a dense sequence: count, and the slot of the r-th.
+struct RangeView {
+  int64_t first;     // slot of the range's start
+  int64_t count;     // particles in the range
+  uint32_t gap_lo;   // unused run [gap_lo, gap_lo + gap_len) relative to first
+  uint32_t gap_len;
+  __host__ __device__ int64_t slot(int64_t r) const {
+    return first + (r < (int64_t)gap_lo ? r : r + gap_len);
+  }
+};
+
+__host__ __device__ inline RangeView range_view(const int64_t *seg, const unsigned int *cnt,
+                                                int ntiles, unsigned int ovf_cap, int t) {
+  RangeView v;
+  v.first = seg[t];
+  const int64_t cap = t < ntiles ? seg[t + 1] - seg[t] : (int64_t)ovf_cap;
+  const int64_t taken = (int64_t)cnt[t] < cap ? (int64_t)cnt[t] : cap;
+  v.gap_lo = cnt[(ntiles + 1) + t];
+  v.gap_len = cnt[2 * (ntiles + 1) + t];
+  v.count = taken - (int64_t)v.gap_len;
+  return v;
+}
+
 template <typename T>
 struct BinnedSpecies {
-  const T *in[7];  // x y z u v w q, binned
-  T *out[7];
+  // x y z u v w q, binned: component c of slot i at in[c * stride + i]
+  const T *in;
+  T *out;
+  int64_t stride;
   const uint32_t *id_in;
   uint32_t *id_out;
   const int64_t *seg;          // [ntiles + 1]
-  const unsigned int *cnt_in;  // [ntiles + 1]
-  unsigned int *cnt_out;       // [ntiles + 1], zero on entry
+  const unsigned int *cnt_in;  // [3 * (ntiles + 1)], see above
+  unsigned int *cnt_out;       // [3 * (ntiles + 1)], zero on entry
   unsigned int ovf_cap;
   KPush<T> k;
   KMom m;
@@ -63,9 +94,10 @@ struct BinnedArgs {
 
 struct BinnedReduceArgs {
   int nyn, nzn;
-  int ntx, nty, ntz;
+  int ntx, nty, ntz, ntiles;
   int ns;
   const double *partials;
+  const int64_t *seg[kMaxSpecies];
   const unsigned int *cnt[kMaxSpecies];
   KMom mom[kMaxSpecies];
 };
@@ -111,8 +143,9 @@ hipError_t zero_counts(const CountArrays &c, hipStream_t s);
 // The source is dense (src_seg == nullptr: particles 0..n-1, ids assigned
 // as their index) or binned (src_seg, src_cnt, src_ovf_cap; ids from
 // src_id).  The destination gets segments of count + slack slots:
-// dst_seg[t] = (particles in tiles < t) + t * slack, dst_cnt the counts,
-// dst_cnt[ntiles] = 0.  With workspace == nullptr only *needed is set.
+// dst_seg[t] = (particles in tiles < t) + t * slack, dst_cnt the counts
+// (no unused runs, an empty overflow region).  With workspace == nullptr
+// only *needed is set.
 template <typename T>
 hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
                     T *const src[7], const uint32_t *src_id,

@@ -10,10 +10,13 @@
 //      - pushes the particle (src/Mover.cpp:33-157) with the eight-node
 //        stencil gathered from LDS and kept in registers while the cell is
 //        unchanged (from global memory if it lies outside the staged block);
-//      - writes it into its destination tile's segment of the output buffer
-//        (one counter atomic per wave for the particles staying in this
-//        tile, one per particle for the others; the overflow region when the
-//        segment is full);
+//      - writes it into the output buffer: a particle that stays in the
+//        tile takes the next slot of a block of its own segment that the
+//        workgroup reserved for this species with one counter atomic (slots
+//        from an LDS counter, one atomic per wave); a particle that left the
+//        tile (it is in the list ring, below) is written when the ring is
+//        flushed, at the next slot of its destination segment's counter;
+//        the overflow region takes particles whose segment is full;
 //      - deposits it (src/Mover.cpp:399-570): a particle in one of the tile's
 //        cells goes into that cell's LDS bucket; one in a halo cell adds its
 //        80 contributions to the LDS moment block; one outside the block adds
@@ -130,15 +133,18 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
         X0 = blk.xn[bx]; X1 = blk.xn[bx + 1];
         Y0 = blk.yn[by]; Y1 = blk.yn[by + 1];
         Z0 = blk.zn[bz]; Z1 = blk.zn[bz + 1];
+        // node (bx + 1 - aa, by + 1 - bb, bz + 1 - cc) lies a compile-time
+        // offset before node (bx + 1, by + 1, bz + 1): one address for all
+        const T *fb = blk.fld + (((bx + 1) * B::BY + (by + 1)) * B::BZ + (bz + 1)) * 6;
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
           for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
-              const int nb = ((bx + 1 - aa) * B::BY + (by + 1 - bb)) * B::BZ + (bz + 1 - cc);
+              const int off = ((aa * B::BY + bb) * B::BZ + cc) * 6;
 #pragma unroll
-              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = blk.fld[nb * 6 + c];
+              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = fb[c - off];
             }
       } else {
         ++n_glob;
@@ -261,10 +267,41 @@ template <typename T>
 __device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, int64_t o,
                                                T x, T y, T z, T u, T v, T w, T q,
                                                uint32_t pid) {
-  sp.out[0][o] = x; sp.out[1][o] = y; sp.out[2][o] = z;
-  sp.out[3][o] = u; sp.out[4][o] = v; sp.out[5][o] = w;
-  sp.out[6][o] = q;
+  T *p = sp.out + o;
+  const int64_t st = sp.stride;
+  p[0] = x; p[st] = y; p[2 * st] = z;
+  p[3 * st] = u; p[4 * st] = v; p[5 * st] = w;
+  p[6 * st] = q;
   sp.id_out[o] = pid;
+}
+
+// Write a pushed particle to the output buffer.  A particle that stays in
+// `tile` takes slot j of the block the workgroup reserved in its own
+// segment (from stay_base); a leaver takes the next slot of its destination
+// segment's counter; when the segment is full, the next slot of the overflow
+// region.  False if that is full too (the particle is lost and the call
+// reports it).
+template <typename T>
+__device__ __forceinline__ bool put_particle(const BinnedArgs<T> &a,
+                                             const BinnedSpecies<T> &sp, int tile,
+                                             unsigned int stay_base, bool stay,
+                                             unsigned int j, int dest, T x, T y, T z,
+                                             T u, T v, T w, T q, uint32_t pid) {
+  int64_t o = -1;
+  if (stay) {
+    const int64_t k = (int64_t)stay_base + j;
+    if (k < sp.seg[tile + 1] - sp.seg[tile]) o = sp.seg[tile] + k;
+  } else {
+    const unsigned int k = atomicAdd(sp.cnt_out + dest, 1u);
+    if ((int64_t)k < sp.seg[dest + 1] - sp.seg[dest]) o = sp.seg[dest] + k;
+  }
+  if (o < 0) {
+    const unsigned int k = atomicAdd(sp.cnt_out + a.ntiles, 1u);
+    if (k < sp.ovf_cap) o = sp.seg[a.ntiles] + (int64_t)k;
+  }
+  if (o < 0) return false;
+  write_particle(sp, o, x, y, z, u, v, w, q, pid);
+  return true;
 }
 
 // The ten moment terms of one node weight, in the reference's operation
@@ -360,7 +397,7 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
 // ask for three waves per SIMD (at most 168 VGPRs).
 template <typename T, typename A, int TX, int TY, int TZ>
 __global__ __launch_bounds__(4 * TX * TY * TZ)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
 void k_binned_phase(BinnedArgs<T> a) {
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
@@ -377,8 +414,11 @@ void k_binned_phase(BinnedArgs<T> a) {
   __shared__ T s_bkt[7][NC * CS];  // x y z q u v w of the bucketed particles
   __shared__ T s_lst[7][NL];       // x y z q u v w of the listed particles
   __shared__ unsigned int s_lcell[NL];  // their deposit cells, packed
+  __shared__ uint32_t s_lid[NL];        // and their ids
   __shared__ unsigned int s_cnt[2][NC];
   __shared__ unsigned int s_nl;    // entries appended to the ring this species
+  __shared__ unsigned int s_stay;  // particles written that stay in the tile
+  __shared__ unsigned int s_base;  // start of the block reserved for them
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -425,16 +465,23 @@ void k_binned_phase(BinnedArgs<T> a) {
 
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const int64_t p0 = sp.seg[tile];
-    const int64_t nin = min((int64_t)sp.cnt_in[tile], sp.seg[tile + 1] - p0);
+    const RangeView in = range_view(sp.seg, sp.cnt_in, a.ntiles, sp.ovf_cap, tile);
+    const int64_t nin = in.count;
     if (nin == 0) continue;  // uniform across the workgroup
     for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
     for (int i = tid; i < 2 * NC; i += NT) (&s_cnt[0][0])[i] = 0u;
-    if (tid == 0) s_nl = 0u;
+    if (tid == 0) {
+      s_nl = 0u;
+      s_stay = 0u;
+      // at most nin particles stay: reserve that many slots of the output
+      // segment in one block
+      s_base = atomicAdd(sp.cnt_out + tile, (unsigned int)nin);
+    }
     // entries [0, lst_done) of the ring have been deposited; the same value
     // in every thread
     unsigned int lst_done = 0u;
     __syncthreads();
+    const unsigned int stay_base = s_base;
     T acc[20];  // register sums, particle precision
 #pragma unroll
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
@@ -447,9 +494,11 @@ void k_binned_phase(BinnedArgs<T> a) {
       T x = T(0), y = T(0), z = T(0), u = T(0), v = T(0), w = T(0), q = T(0);
       uint32_t pid = 0;
       if (active) {
-        const int64_t i = p0 + r;
-        x = sp.in[0][i]; y = sp.in[1][i]; z = sp.in[2][i];
-        u = sp.in[3][i]; v = sp.in[4][i]; w = sp.in[5][i]; q = sp.in[6][i];
+        const int64_t i = in.slot(r);
+        const T *p = sp.in + i;
+        const int64_t st = sp.stride;
+        x = p[0]; y = p[st]; z = p[2 * st];
+        u = p[3 * st]; v = p[4 * st]; w = p[5 * st]; q = p[6 * st];
         pid = sp.id_in[i];
       }
       int ix = 2, iy = 2, iz = 2, dest = -1;
@@ -465,29 +514,9 @@ void k_binned_phase(BinnedArgs<T> a) {
         dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
       }
 
-      // re-bin: one counter atomic per wave for the particles that stay
-      {
-        const bool stay = active && dest == tile;
-        const unsigned long long stay_mask = __ballot(stay);
-        unsigned int slot = 0;
-        if (stay_mask) {
-          const int leader = __ffsll((long long)stay_mask) - 1;
-          unsigned int wbase = 0;
-          if (lane == leader)
-            wbase = atomicAdd(sp.cnt_out + tile, (unsigned int)__popcll(stay_mask));
-          wbase = __shfl(wbase, leader);
-          if (stay)
-            slot = wbase + (unsigned int)__popcll(stay_mask & ((1ull << lane) - 1ull));
-        }
-        if (active && !stay) slot = atomicAdd(sp.cnt_out + dest, 1u);
-        if (active) {
-          const int64_t o = out_slot(a, sp, dest, slot);
-          if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
-          else bad_store = true;
-        }
-      }
-
-      // deposit: bucket (tile cell), LDS atomics (halo cell), global
+      // deposit: bucket (tile cell), else the list ring (halo cell,
+      // outside the block, full bucket), else directly
+      bool bucketed = false;
       if (active) {
         const int lcx = ix - 2 - blk.c0x, lcy = iy - 2 - blk.c0y, lcz = iz - 2 - blk.c0z;
         const bool in_tile = (lcx >= 0) & (lcx < TX) & (lcy >= 0) & (lcy < TY) &
@@ -501,6 +530,7 @@ void k_binned_phase(BinnedArgs<T> a) {
             s_bkt[0][e] = x; s_bkt[1][e] = y; s_bkt[2][e] = z; s_bkt[3][e] = q;
             s_bkt[4][e] = u; s_bkt[5][e] = v; s_bkt[6][e] = w;
             done = true;
+            bucketed = true;
             ++n_ld;
           }
         }
@@ -512,6 +542,7 @@ void k_binned_phase(BinnedArgs<T> a) {
             s_lst[0][e] = x; s_lst[1][e] = y; s_lst[2][e] = z; s_lst[3][e] = q;
             s_lst[4][e] = u; s_lst[5][e] = v; s_lst[6][e] = w;
             s_lcell[e] = ((unsigned int)ix << 20) | ((unsigned int)iy << 10) | (unsigned int)iz;
+            s_lid[e] = pid;
             done = true;
           }
         }
@@ -526,6 +557,27 @@ void k_binned_phase(BinnedArgs<T> a) {
                                                  u, v, w, q);
           if (lds) ++n_ld;
           else ++n_gd;
+          const bool stay = dest == tile;
+          const unsigned int jj = stay ? atomicAdd(&s_stay, 1u) : 0u;
+          bad_store |= !put_particle(a, sp, tile, stay_base, stay, jj, dest, x, y, z, u, v,
+                                     w, q, pid);
+        }
+      }
+
+      // the bucketed particles stay in the tile: their slots come from the
+      // workgroup's reserved block, one LDS counter atomic per wave
+      {
+        const unsigned long long m = __ballot(bucketed);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          unsigned int jb = 0;
+          if (lane == leader) jb = atomicAdd(&s_stay, (unsigned int)__popcll(m));
+          jb = __shfl(jb, leader);
+          if (bucketed) {
+            const unsigned int jj = jb + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+            bad_store |= !put_particle(a, sp, tile, stay_base, true, jj, tile, x, y, z, u,
+                                       v, w, q, pid);
+          }
         }
       }
 
@@ -588,6 +640,12 @@ void k_binned_phase(BinnedArgs<T> a) {
             if (nd == 0) {
               if (lds) ++n_ld;
               else ++n_gd;
+              // and the entry's particle to the output buffer
+              const int pdest = tile_of<TX, TY, TZ>(a, jx, jy, jz);
+              const bool stay = pdest == tile;
+              const unsigned int jj = stay ? atomicAdd(&s_stay, 1u) : 0u;
+              bad_store |= !put_particle(a, sp, tile, stay_base, stay, jj, pdest, px, py,
+                                         pz, pu, pv, pw, pq, s_lid[e]);
             }
           }
           lst_done = nl;
@@ -610,6 +668,18 @@ void k_binned_phase(BinnedArgs<T> a) {
     __syncthreads();
     double *dst = a.partials + ((size_t)tile * a.ns + s) * (10 * NB);
     for (int i = tid; i < 10 * NB; i += NT) dst[i] = (double)s_mom[i];
+    if (tid == 0) {
+      // the reserved block [stay_base, stay_base + nin) holds s_stay
+      // particles; its remainder inside the segment stays unused
+      const int64_t cap = sp.seg[tile + 1] - sp.seg[tile];
+      const int64_t room = cap > (int64_t)stay_base ? cap - (int64_t)stay_base : 0;
+      const int64_t lo = (int64_t)stay_base + min((int64_t)s_stay, room);
+      const int64_t hi = min((int64_t)stay_base + nin, cap);
+      if (hi > lo) {
+        sp.cnt_out[(a.ntiles + 1) + tile] = (unsigned int)lo;
+        sp.cnt_out[2 * (a.ntiles + 1) + tile] = (unsigned int)(hi - lo);
+      }
+    }
     __syncthreads();
   }
   if (bad_move) atomicAdd(a.status, 1u);
@@ -635,14 +705,16 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
   unsigned int n_gg = 0, n_gd = 0, n_lg = 0;
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const int64_t p0 = sp.seg[a.ntiles];
-    const int64_t nin = min((int64_t)sp.cnt_in[a.ntiles], (int64_t)sp.ovf_cap);
+    const RangeView ov = range_view(sp.seg, sp.cnt_in, a.ntiles, sp.ovf_cap, a.ntiles);
+    const int64_t nin = ov.count;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nin;
          r += (int64_t)gridDim.x * blockDim.x) {
-      const int64_t i = p0 + r;
-      T x = sp.in[0][i], y = sp.in[1][i], z = sp.in[2][i];
-      T u = sp.in[3][i], v = sp.in[4][i], w = sp.in[5][i];
-      const T q = sp.in[6][i];
+      const int64_t i = ov.slot(r);
+      const T *p = sp.in + i;
+      const int64_t st = sp.stride;
+      T x = p[0], y = p[st], z = p[2 * st];
+      T u = p[3 * st], v = p[4 * st], w = p[5 * st];
+      const T q = p[6 * st];
       const uint32_t pid = sp.id_in[i];
       push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
                                        bad_move, n_lg, n_gg);

@@ -42,30 +42,28 @@ __device__ __forceinline__ int phys_cell(T p, double start, T inv, int nc) {
   return (int)c;
 }
 
-// Particles of a binned source that a dense enumeration can reach: the
-// count of each segment clamped to its capacity (the excess was written to
-// the overflow region), and of the overflow region to its capacity.
-__global__ void k_clamped_counts(const int64_t *__restrict__ seg,
-                                 const unsigned int *__restrict__ cnt,
-                                 unsigned int ovf_cap, int ntiles,
-                                 unsigned int *__restrict__ out) {
+// Particles per range of a binned source (tiles, then the overflow region).
+__global__ void k_range_counts(const int64_t *__restrict__ seg,
+                               const unsigned int *__restrict__ cnt,
+                               unsigned int ovf_cap, int ntiles,
+                               unsigned int *__restrict__ out) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
-       t += gridDim.x * blockDim.x) {
-    const int64_t cap = t < ntiles ? seg[t + 1] - seg[t] : (int64_t)ovf_cap;
-    out[t] = (unsigned int)min((int64_t)cnt[t], cap);
-  }
+       t += gridDim.x * blockDim.x)
+    out[t] = (unsigned int)range_view(seg, cnt, ntiles, ovf_cap, t).count;
 }
 
-// Source slot of dense particle i: the last range t with P[t] <= i.
+// Source slot of dense particle i: its range is the last t with P[t] <= i.
 __device__ __forceinline__ int64_t source_slot(int64_t i, const unsigned int *P,
-                                               const int64_t *seg, int ntiles) {
+                                               const int64_t *seg,
+                                               const unsigned int *cnt,
+                                               unsigned int ovf_cap, int ntiles) {
   int lo = 0, hi = ntiles;  // answer in [0, ntiles]
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if ((int64_t)P[mid] <= i) lo = mid;
     else hi = mid - 1;
   }
-  return seg[lo] + (i - (int64_t)P[lo]);
+  return range_view(seg, cnt, ntiles, ovf_cap, lo).slot(i - (int64_t)P[lo]);
 }
 
 template <typename T>
@@ -74,12 +72,13 @@ __global__ void k_bin_keys(KGrid<T> g, BinLayout L, int64_t n,
                            const T *__restrict__ z,
                            const unsigned int *__restrict__ P,
                            const int64_t *__restrict__ seg,
+                           const unsigned int *__restrict__ cnt, unsigned int ovf_cap,
                            unsigned int *__restrict__ cell_count,
                            uint32_t *__restrict__ key, uint32_t *__restrict__ idx) {
   const uint32_t rank_max = (1u << L.rank_bits) - 1u;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = P ? source_slot(i, P, seg, L.ntiles) : i;
+    const int64_t j = P ? source_slot(i, P, seg, cnt, ovf_cap, L.ntiles) : i;
     const int cx = phys_cell<T>(x[j], g.xStart, g.invdx, L.ncx);
     const int cy = phys_cell<T>(y[j], g.yStart, g.invdy, L.ncy);
     const int cz = phys_cell<T>(z[j], g.zStart, g.invdz, L.ncz);
@@ -122,7 +121,7 @@ __global__ void k_bin_gather(Arrays7<T> src, const uint32_t *__restrict__ src_id
 }
 
 // seg[t] = (first sorted position of tile >= t) + t * slack; cnt from the
-// differences; the overflow region starts empty.
+// differences, no unused runs; the overflow region starts empty.
 __global__ void k_bin_bounds(const uint32_t *__restrict__ keys, int shift, int64_t n,
                              int ntiles, int64_t slack, int64_t *__restrict__ seg,
                              unsigned int *__restrict__ cnt) {
@@ -141,6 +140,8 @@ __global__ void k_bin_bounds(const uint32_t *__restrict__ keys, int shift, int64
     }
     seg[t] = b[0] + (int64_t)t * slack;
     cnt[t] = t < ntiles ? (unsigned int)(b[1] - b[0]) : 0u;
+    cnt[(ntiles + 1) + t] = 0u;
+    cnt[2 * (ntiles + 1) + t] = 0u;
   }
 }
 
@@ -150,12 +151,9 @@ __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
                             const int64_t *__restrict__ seg,
                             const unsigned int *__restrict__ cnt,
                             unsigned int ovf_cap, int ntiles, Arrays7<T> dst) {
-  const int t = blockIdx.x;
-  const int64_t p0 = seg[t];
-  const int64_t cap = t < ntiles ? seg[t + 1] - p0 : (int64_t)ovf_cap;
-  const int64_t m = min((int64_t)cnt[t], cap);
-  for (int64_t r = threadIdx.x; r < m; r += blockDim.x) {
-    const int64_t i = p0 + r;
+  const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
+  for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
+    const int64_t i = v.slot(r);
     const uint32_t k = id[i];
 #pragma unroll
     for (int a = 0; a < 7; ++a) dst.p[a][k] = src.p[a][i];
@@ -190,6 +188,7 @@ __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int n
   const int zl = gz >= BZ - 1 ? (gz - (BZ - 1) + TZ - 1) / TZ : 0;
   const int zh = min(gz / TZ, a.ntz - 1);
   const unsigned int *cnt = a.cnt[s];
+  const int64_t *seg = a.seg[s];
   double sum[10], cmp[10];
 #pragma unroll
   for (int m = 0; m < 10; ++m) { sum[m] = 0.0; cmp[m] = 0.0; }
@@ -198,7 +197,7 @@ __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int n
     for (int ty = yl; ty <= yh; ++ty)
       for (int tz = zl; tz <= zh; ++tz) {
         const int tile = (tx * a.nty + ty) * a.ntz + tz;
-        if (cnt[tile] == 0u) continue;
+        if (range_view(seg, cnt, a.ntiles, 0u, tile).count == 0) continue;
         any = true;
         const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
         const double *src = a.partials + ((size_t)tile * a.ns + s) * (10 * NB) + b;
@@ -278,7 +277,7 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
   void *scan_tmp = (void *)(w + 4 * a_n + 2 * a_t + a_c + align_up(sort_bytes));
 
   if (binned && n > 0) {
-    hipLaunchKernelGGL(k_clamped_counts, dim3(nblocks((long)nt1)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_range_counts, dim3(nblocks((long)nt1)), dim3(256), 0, s,
                        src_seg, src_cnt, src_ovf_cap, L.ntiles, clamped);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = rocprim::exclusive_scan(scan_tmp, scan_bytes, (const unsigned int *)clamped,
@@ -291,8 +290,8 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_keys<T>, dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
                        (const T *)src[0], (const T *)src[1], (const T *)src[2],
-                       binned ? (const unsigned int *)P : nullptr, src_seg, cell_count,
-                       keys_in, idx_in);
+                       binned ? (const unsigned int *)P : nullptr, src_seg, src_cnt,
+                       src_ovf_cap, cell_count, keys_in, idx_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, idx_in,
                                   perm, (size_t)n, 0, key_bits, s);

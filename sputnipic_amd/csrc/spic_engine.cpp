@@ -33,9 +33,10 @@ struct spic_engine {
     int64_t slack = 0;             // free slots per segment after a sort
     int64_t cap = 0;               // slots per buffer
     unsigned int ovf_cap = 0;      // overflow slots (= n: nothing is ever lost)
-    void *buf[2][7] = {};          // two buffers of x y z u v w q
+    void *block[2] = {};           // two buffers of x y z u v w q, one block each
+    void *buf[2][7] = {};          // the components: block[b] + c * cap
     uint32_t *id[2] = {nullptr, nullptr};
-    unsigned int *cnt[2] = {nullptr, nullptr};  // [ntiles + 1] per buffer
+    unsigned int *cnt[2] = {nullptr, nullptr};  // [3 * (ntiles + 1)] per buffer
     int64_t *seg[2] = {nullptr, nullptr};       // [ntiles + 1] per layout
     int cur = 0;                   // buffer holding the particles
     int segi = 0;                  // layout in use
@@ -66,8 +67,7 @@ hipError_t dmalloc(spic_engine *e, void **p, size_t bytes) {
 
 void free_species(spic_engine *e, spic_engine::Species &s) {
   for (int b = 0; b < 2; ++b) {
-    for (int a = 0; a < 7; ++a)
-      if (s.buf[b][a]) (void)hipFree(s.buf[b][a]);
+    if (s.block[b]) (void)hipFree(s.block[b]);
     if (s.id[b]) (void)hipFree(s.id[b]);
     if (s.cnt[b]) (void)hipFree(s.cnt[b]);
     if (s.seg[b]) (void)hipFree(s.seg[b]);
@@ -160,10 +160,9 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     idx[k] = is;
     auto &t = a.sp[k];
     const int in = sp.cur, out = 1 - sp.cur;
-    for (int c = 0; c < 7; ++c) {
-      t.in[c] = (const T *)sp.buf[in][c];
-      t.out[c] = (T *)sp.buf[out][c];
-    }
+    t.in = (const T *)sp.block[in];
+    t.out = (T *)sp.block[out];
+    t.stride = sp.cap;
     t.id_in = sp.id[in];
     t.id_out = sp.id[out];
     t.seg = sp.seg[sp.segi];
@@ -178,13 +177,14 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     t.k = spic::kpush<T>(&desc, push);
     t.m = spic::kmom(&mom[is]);
     r.cnt[k] = sp.cnt[in];
+    r.seg[k] = sp.seg[sp.segi];
     r.mom[k] = t.m;
     zc.p[k] = sp.cnt[out];
     ++k;
   }
   if (k == 0) return hipSuccess;
   a.ns = r.ns = zc.ns = k;
-  zc.n = e->L.ntiles + 1;
+  zc.n = 3 * (e->L.ntiles + 1);
   hipError_t err = spic::zero_counts(zc, s);
   if (err != hipSuccess) return err;
   err = spic::strict_mode() ? spic::spic_strict::launch_binned_phase<T>(a, s)
@@ -192,6 +192,7 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   if (err != hipSuccess) return err;
   r.nyn = e->grd.nyn; r.nzn = e->grd.nzn;
   r.ntx = e->L.ntx; r.nty = e->L.nty; r.ntz = e->L.ntz;
+  r.ntiles = e->L.ntiles;
   r.partials = e->partials;
   const int nn = e->grd.nxn * e->grd.nyn * e->grd.nzn;
   err = spic::launch_binned_reduce(r, e->L.tx, e->L.ty, e->L.tz, nn, s);
@@ -325,12 +326,13 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
     if ((err = hipDeviceSynchronize()) != hipSuccess) return hip_fail(err, "engine load");
     free_species(e, sp);
     for (int b = 0; b < 2; ++b) {
+      if ((err = dmalloc(e, &sp.block[b], (size_t)e->esz * 7 * cap)) != hipSuccess)
+        return hip_fail(err, "engine particle storage");
       for (int a = 0; a < 7; ++a)
-        if ((err = dmalloc(e, &sp.buf[b][a], (size_t)e->esz * cap)) != hipSuccess)
-          return hip_fail(err, "engine particle storage");
+        sp.buf[b][a] = (char *)sp.block[b] + (size_t)e->esz * a * cap;
       if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
         return hip_fail(err, "engine id storage");
-      if ((err = dmalloc(e, (void **)&sp.cnt[b], sizeof(unsigned int) * (nt + 1))) !=
+      if ((err = dmalloc(e, (void **)&sp.cnt[b], sizeof(unsigned int) * 3 * (nt + 1))) !=
               hipSuccess ||
           (err = dmalloc(e, (void **)&sp.seg[b], sizeof(int64_t) * (nt + 1))) !=
               hipSuccess)
@@ -451,7 +453,7 @@ int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
     err = hipMemcpy(&ovf, sp.cnt[sp.cur] + e->L.ntiles, sizeof(ovf),
                     hipMemcpyDeviceToHost);
     if (err != hipSuccess) return hip_fail(err, "engine overflow count");
-    st->overflow += ovf;
+    st->overflow += ovf < sp.ovf_cap ? ovf : sp.ovf_cap;
   }
   return SPIC_OK;
 }
