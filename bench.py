@@ -331,6 +331,159 @@ def run_stress(warmup: int) -> dict:
     }
 
 
+BATCH_INSTANCES = 16
+BATCH_SEED = 3
+BATCH_SCALES = (2, 3, 4, 5, 6)  # grid (16 s) x (8 s) x (8 s): 32x16x16 .. 96x48x48
+# (npcelx, npcely): the generator's lattice loop runs its z index to npcely,
+# as the reference's does (src/IC.cpp:128), so a cell holds npcelx * npcely^2
+# particles: 1 .. 64
+BATCH_PPC = tuple((px, py) for px in range(1, 5) for py in range(1, 5))
+BATCH_N_RANGE = (500_000, 60_000_000)
+
+
+def batch_specs(seed: int = BATCH_SEED):
+    """The 16 independent Harris-sheet instances of the batching form: per
+    instance a grid scale, a lattice per species (1..64 particles per cell),
+    drawn again until the instance holds 0.5M..60M particles, a
+    perturbation amplitude in [0.005, 0.05] and its own generator seed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(BATCH_INSTANCES):
+        while True:
+            sc = int(rng.choice(BATCH_SCALES))
+            grid = (16 * sc, 8 * sc, 8 * sc)
+            ppc = [BATCH_PPC[int(rng.integers(len(BATCH_PPC)))] for _ in range(NS)]
+            cells = grid[0] * grid[1] * grid[2]
+            n = sum(cells * px * py * py for px, py in ppc)
+            if BATCH_N_RANGE[0] <= n <= BATCH_N_RANGE[1]:
+                break
+        specs.append({"grid": grid, "ppc": ppc, "pertX": float(rng.uniform(0.005, 0.05)),
+                      "seed": k + 1, "particles": n})
+    return specs
+
+
+class GemInstance:
+    """One instance of the batching form, generated on the device: FP32
+    fields and particles, an engine with the given tile shape."""
+
+    def __init__(self, spec: dict, tile, stream):
+        import sputnipic_amd as S
+        self.grd = S.Grid(*spec["grid"], *BOX, "f32")
+        gem = S.GemParams(pertX=spec["pertX"])
+        self.field = S.gem_fields(self.grd, gem)
+        species = S.harris_species()
+        for sp, (px, py) in zip(species, spec["ppc"]):
+            sp.npcelx, sp.npcely, sp.npcelz = px, py, py
+        self.eng = S.Engine(self.grd, len(species), tile=tile)
+        self.n = 0
+        for s, sp in enumerate(species):
+            p = S.gem_particles(self.grd, gem, sp, s, seed=spec["seed"])
+            self.n += p.npmax
+            self.eng.load(s, p, stream)
+            stream.synchronize()
+            for d in p.d.values():
+                d.free()
+        self.ids = [S.InterpDensSpecies(self.grd) for _ in species]
+        self.param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
+        self.ns = len(species)
+
+    def member(self):
+        return (self.eng, self.field, self.ids, self.param)
+
+    def zero(self, stream):
+        import sputnipic_amd as S
+        S.setZeroSpeciesDensities(self.ids, self.grd, self.ns, stream)
+
+    def close(self):
+        self.eng.close()
+        for m in self.ids:
+            for d in m.d.values():
+                d.free()
+
+
+def run_batch(steps: int, warmup: int) -> dict:
+    """The batching form: 16 independent Harris-sheet instances (batch_specs)
+    in FP32, advanced by one batch (one launch per kernel and phase); tile
+    shape and sort schedule from the autotuner, timed on the same instances
+    built afresh; then `steps` cycles (zero every instance's moments, batch
+    phase) captured in one HIP graph and timed as one launch."""
+    import sputnipic_amd as S
+    from sputnipic_amd.tune import tune_batch
+
+    specs = batch_specs()
+    n_total = sum(sp["particles"] for sp in specs)
+    log(f"batch: {len(specs)} instances, {n_total} particles, grids "
+        f"{sorted(set(sp['grid'] for sp in specs))}")
+    stream = S.Stream()
+    t0 = time.perf_counter()
+    tuned = tune_batch(lambda tile: [GemInstance(sp, tile, stream) for sp in specs],
+                       phases=6, warmup=1, stream=stream)
+    t_tune = time.perf_counter() - t0
+    log(f"batch: tuned in {t_tune:.1f} s: " + ", ".join(
+        f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}: {t.ms_per_phase:.3f} ms"
+        for t in tuned.trials))
+    insts = [GemInstance(sp, tuned.tile, stream) for sp in specs]
+    assert sum(x.n for x in insts) == n_total
+    batch = S.Batch([x.member() for x in insts], sort_every=tuned.sort_every)
+
+    def cycle():
+        for x in insts:
+            x.zero(stream)
+        batch.phase(stream)
+
+    S.set_async(True)
+    try:
+        for _ in range(warmup):
+            cycle()
+        stream.synchronize()
+        S.check_domain(stream)
+        S.set_async(True)
+        with S.Graph(stream) as graph:
+            for _ in range(steps):
+                cycle()
+        a, b = S.Event(), S.Event()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        a.record(stream)
+        graph.launch()
+        b.record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        S.check_domain(stream)
+    finally:
+        S.set_async(False)
+    gpu_ms = a.elapsed_ms(b)
+    stats = [x.eng.stats() for x in insts]
+    graph.close()
+    batch.close()
+    for x in insts:
+        x.close()
+    per_cycle_bytes = sum(algorithmic_bytes(x.n, x.grd.nn, 4, x.ns) for x in insts)
+    value = n_total * steps / wall
+    achieved = per_cycle_bytes * steps / (gpu_ms * 1e-3) / 1e9
+    log(f"batch: {value / 1e9:.3f} G particles/s, {1e3 * wall / steps:.3f} ms/cycle "
+        f"({gpu_ms / steps:.3f} ms/cycle on the GPU clock), {achieved:.1f} GB/s "
+        f"({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
+    return {
+        "value": value,
+        "ms_per_step": 1e3 * wall / steps,
+        "gpu_ms_per_cycle": gpu_ms / steps,
+        "particles": n_total,
+        "instances": [{"grid": "x".join(map(str, sp["grid"])),
+                       "ppc": [px * py * py for px, py in sp["ppc"]],
+                       "pertX": round(sp["pertX"], 5), "particles": sp["particles"]}
+                      for sp in specs],
+        "tuned": tuned.as_dict(),
+        "tune_seconds": t_tune,
+        "achieved_GBs": achieved,
+        "roofline_frac": achieved / HBM_PEAK_GBS,
+        "bytes_per_cycle": per_cycle_bytes,
+        "overflow": sum(st["overflow"] for st in stats),
+        "global_deposits": sum(st["global_deposits"] for st in stats),
+    }
+
+
 def form_errors(m32, m64, interior: bool = False) -> dict:
     """FP32 form against FP64 form after the same cycles from the same
     initial condition: per moment, max |m32 - m64| over all species and
@@ -453,9 +606,11 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
-    ap.add_argument("--workload", choices=("gem", "stress"), default="gem",
+    ap.add_argument("--workload", choices=("gem", "stress", "batch"), default="gem",
                     help="gem: the 262M-particle Harris sheet (default); stress: 8 "
-                         "species, 314.6M particles, FP32, 50 cycles in one HIP graph")
+                         "species, 314.6M particles, FP32, 50 cycles in one HIP graph; "
+                         "batch: 16 independent Harris-sheet instances in FP32, one "
+                         "launch per kernel and phase, autotuned, in one HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once device to device (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -514,6 +669,45 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": f["bytes_per_cycle"],
             },
             "stress": f,
+            "cpu_baseline": None,
+        }
+        print(json.dumps(line), flush=True)
+        return 0
+    if args.workload == "batch":
+        f = run_batch(args.steps, args.warmup)
+        line = {
+            "metric": METRIC,
+            "value": f["value"],
+            "unit": "particles/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": f["ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {
+                "workload": "batch: 16 independent Harris-sheet GEM 3D instances, grids "
+                            "32x16x16..96x48x48 cells on a 20x10x10 box, 4 species at 1..64 "
+                            "particles per cell (0.5M..60M per instance), pertX in "
+                            "[0.005, 0.05], FP32, one launch per kernel and phase, tile "
+                            "shape and sort schedule autotuned, cycles in one HIP graph",
+                "particles": f["particles"],
+            },
+            "roofline": {
+                "bound": "hbm",
+                "achieved": f["achieved_GBs"],
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": f["roofline_frac"],
+                "traffic": None,
+                "kernel": "whole captured cycle (zero, batched phase, overflow, reduce, "
+                          "sorts)",
+                "algorithmic_bytes_per_launch": f["bytes_per_cycle"],
+            },
+            "batch": f,
             "cpu_baseline": None,
         }
         print(json.dumps(line), flush=True)

@@ -227,6 +227,40 @@ int spic_engine_sort(spic_engine *eng, void *stream);
 int spic_engine_get_stats(const spic_engine *eng, spic_engine_stats *st);
 
 /* ---------------------------------------------------------------------- */
+/* Batches: up to SPIC_MAX_INSTANCES independent PIC instances (engines    */
+/* with their own grids, fields, species and moments) advanced together,  */
+/* one launch per kernel and phase: every instance's tiles form one grid   */
+/* of the tile kernel, and the count reset, overflow pass and node reduce  */
+/* are one launch each.  The reference runs independent simulations as    */
+/* separate processes; this is the same work in one stream of launches.    */
+/* Each instance's results equal those of spic_engine_phase on its own.    */
+/* Members share precision, tile shape and LDS partials mode.  The field,  */
+/* moment and push descriptors are copied at creation (the arrays they     */
+/* point to must stay allocated).  Load every species before creating the  */
+/* batch; afterwards advance the members only through spic_batch_phase     */
+/* (a load re-reads the members at the next phase, outside graph capture). */
+/* ---------------------------------------------------------------------- */
+#define SPIC_MAX_INSTANCES 16
+typedef struct spic_batch spic_batch;
+
+typedef struct spic_batch_member {
+  spic_engine *engine;
+  const spic_field *field;
+  const spic_moments *moments; /* one set per engine species */
+  const spic_push *push;
+} spic_batch_member;
+
+/* Bind n (1..SPIC_MAX_INSTANCES) distinct engines.  sort_every > 0 sorts
+ * every species of every member before every sort_every-th phase; the
+ * members' own sort_every does not apply to batch phases. */
+int spic_batch_create(spic_batch **batch, const spic_batch_member *members,
+                      int n, int sort_every);
+/* One particle phase of every member: mover_PC then interpP2G of each
+ * species, ADDING the moments into the bound moment sets. */
+int spic_batch_phase(spic_batch *batch, void *stream);
+int spic_batch_destroy(spic_batch *batch);
+
+/* ---------------------------------------------------------------------- */
 /* Synthetic Harris-sheet (GEM) input on the device, for runs too large    */
 /* for a CPU initial condition.  Restates initGEM (src/IC.cpp:38-177) and  */
 /* interpN2Crho (src/InterpDensSpecies.cpp:64-77) with one change: the     */

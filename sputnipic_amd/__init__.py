@@ -7,10 +7,10 @@ The compute lives in ``lib/libspic.so`` (hand-written HIP, C ABI in
 reference's interface over that ABI.  There is no CPU fallback.
 """
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
-from .core import (DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams, Graph,  # noqa: F401
+from .core import (Batch, DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams, Graph,  # noqa: F401
                    GemSpecies, Grid, InterpDensNet, InterpDensSpecies, Parameters,
                    Particles, Stream, FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, applyBCids,
-                   applyBCids_all, beam_species, check_domain, device_count, gem_fields, gem_particles, harris_species,
+                   applyBCids_all, beam_species, check_domain, device_count, gem_fields, gem_particles, get_async, harris_species,
                    interpP2G, mover_interp, mover_PC, particle_sort, set_async,
                    set_strict_fp, setZeroSpeciesDensities, sumOverSpecies)
 

@@ -95,6 +95,11 @@ class spic_engine_stats(C.Structure):
                 ("overflow", C.c_int64)]
 
 
+class spic_batch_member(C.Structure):
+    _fields_ = [("engine", _P), ("field", C.POINTER(spic_field)),
+                ("moments", C.POINTER(spic_moments)), ("push", C.POINTER(spic_push))]
+
+
 # name -> (restype, argtypes); every entry point of include/spic.h
 SIGNATURES = {
     "spic_mover_pc": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_field),
@@ -120,6 +125,10 @@ SIGNATURES = {
                                     C.POINTER(spic_push), _P]),
     "spic_engine_sort": (C.c_int, [_P, _P]),
     "spic_engine_get_stats": (C.c_int, [_P, C.POINTER(spic_engine_stats)]),
+    "spic_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_batch_member), C.c_int,
+                                    C.c_int]),
+    "spic_batch_phase": (C.c_int, [_P, _P]),
+    "spic_batch_destroy": (C.c_int, [_P]),
     "spic_gem_fields": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_gem_params),
                                   C.POINTER(spic_field), _P]),
     "spic_gem_particles": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_gem_params),

@@ -111,6 +111,27 @@ class Parameters:
                          int(self.PERIODICZ))
 
 
+def grid_constants(nxc: int, nyc: int, nzc: int, Lx: float, Ly: float, Lz: float,
+                   prec: str = "f64") -> dict:
+    """setGrid's spacings, FPfield inverses and node coordinates for nxc x
+    nyc x nzc physical cells (src/Grid.cpp:5-62): dx = Lx / nxc in double;
+    invdx = (FPfield)1.0 / dx and invVOL = (FPfield)1.0 / (dx * dy * dz),
+    divided in double and narrowed on assignment; node i at
+    xStart + (i - 1) * dx (double) narrowed to FPfield, i = 0 .. nxc + 2."""
+    dt = _dtype(prec)
+    dx, dy, dz = Lx / nxc, Ly / nyc, Lz / nzc
+    return {
+        "dx": dx, "dy": dy, "dz": dz,
+        "invVOL": float(dt(1.0 / (dx * dy * dz))),
+        "invdx": float(dt(1.0 / dx)),
+        "invdy": float(dt(1.0 / dy)),
+        "invdz": float(dt(1.0 / dz)),
+        "xn": np.array([(i - 1) * dx for i in range(nxc + 3)], dtype=dt),
+        "yn": np.array([(j - 1) * dy for j in range(nyc + 3)], dtype=dt),
+        "zn": np.array([(k - 1) * dz for k in range(nzc + 3)], dtype=dt),
+    }
+
+
 class Grid:
     """``setGrid`` (src/Grid.cpp:5-62) for nxc x nyc x nzc physical cells on
     an Lx x Ly x Lz box: two guard cells per axis, nodes = cells + 1, node
@@ -120,22 +141,15 @@ class Grid:
     def __init__(self, nxc: int, nyc: int, nzc: int, Lx: float, Ly: float, Lz: float,
                  prec: str = "f64"):
         self.prec = prec
-        dt = _dtype(prec)
         self.nxc, self.nyc, self.nzc = nxc + 2, nyc + 2, nzc + 2
         self.nxn, self.nyn, self.nzn = self.nxc + 1, self.nyc + 1, self.nzc + 1
-        self.dx, self.dy, self.dz = Lx / nxc, Ly / nyc, Lz / nzc
-        self.invVOL = float(dt(dt(1.0) / (self.dx * self.dy * self.dz)))
-        self.invdx = float(dt(dt(1.0) / self.dx))
-        self.invdy = float(dt(dt(1.0) / self.dy))
-        self.invdz = float(dt(dt(1.0) / self.dz))
+        k = grid_constants(nxc, nyc, nzc, Lx, Ly, Lz, prec)
+        self.dx, self.dy, self.dz = k["dx"], k["dy"], k["dz"]
+        self.invVOL = k["invVOL"]
+        self.invdx, self.invdy, self.invdz = k["invdx"], k["invdy"], k["invdz"]
         self.xStart = self.yStart = self.zStart = 0.0
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
-        self.xn_host = np.array([self.xStart + (i - 1) * self.dx for i in range(self.nxn)],
-                                dtype=dt)
-        self.yn_host = np.array([self.yStart + (j - 1) * self.dy for j in range(self.nyn)],
-                                dtype=dt)
-        self.zn_host = np.array([self.zStart + (k - 1) * self.dz for k in range(self.nzn)],
-                                dtype=dt)
+        self.xn_host, self.yn_host, self.zn_host = k["xn"], k["yn"], k["zn"]
         self._xn = DeviceArray.from_numpy(self.xn_host)
         self._yn = DeviceArray.from_numpy(self.yn_host)
         self._zn = DeviceArray.from_numpy(self.zn_host)
@@ -228,8 +242,17 @@ def set_strict_fp(strict: bool) -> None:
     check(lib().spic_set_strict_fp(int(bool(strict))), "spic_set_strict_fp")
 
 
+_ASYNC = [False]
+
+
 def set_async(async_mode: bool) -> None:
     check(lib().spic_set_async(int(bool(async_mode))), "spic_set_async")
+    _ASYNC[0] = bool(async_mode)
+
+
+def get_async() -> bool:
+    """The mode last set through set_async (False at start)."""
+    return _ASYNC[0]
 
 
 def check_domain(stream: Optional["Stream"] = None) -> None:
@@ -526,6 +549,49 @@ class Engine:
         if getattr(self, "ptr", None):
             lib().spic_engine_destroy(self.ptr)
             self.ptr = None
+
+
+class Batch:
+    """Independent PIC instances advanced together (spic_batch_*): one
+    launch per kernel and phase over every instance's tiles.
+
+    members: (engine, field, moment sets, parameters) per instance, the
+    engines' species already loaded.  The descriptors are bound here; the
+    arrays behind them must outlive the batch."""
+
+    def __init__(self, members, sort_every: int = 0):
+        from ._capi import spic_batch_member
+        n = len(members)
+        self._keep = []
+        arr = (spic_batch_member * n)()
+        for i, (eng, field, ids, param) in enumerate(members):
+            f = field.c_struct()
+            moms = (spic_moments * eng.ns)(*[ids[j].c_struct() for j in range(eng.ns)])
+            p = param.push()
+            self._keep += [f, moms, p]
+            arr[i].engine = eng.ptr
+            arr[i].field = C.pointer(f)
+            arr[i].moments = C.cast(moms, C.POINTER(spic_moments))
+            arr[i].push = C.pointer(p)
+        self.engines = [m[0] for m in members]
+        ptr = C.c_void_p()
+        check(lib().spic_batch_create(C.byref(ptr), arr, n, sort_every), "spic_batch_create")
+        self.ptr = ptr.value
+
+    def phase(self, stream: Optional["Stream"] = None) -> None:
+        check(lib().spic_batch_phase(self.ptr, stream.ptr if stream else None),
+              "spic_batch_phase")
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().spic_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __del__(self):
         try:

@@ -28,6 +28,32 @@ namespace spic {
 constexpr int kMaxSpecies = 8;
 constexpr int kBucketCap = 7;  // LDS bucket slots per cell and batch
 constexpr int kListCap = 128;  // LDS list ring entries
+constexpr int kMaxInstances = 16;  // engines in one batched launch
+
+// Blocks of a batched launch: instance i owns blocks [start[i], start[i+1]),
+// i < n; start[n] is the grid size.
+struct BatchMap {
+  int start[kMaxInstances + 1];
+  int n;
+};
+
+// The argument blocks hold generic pointers.  Read from a device table (the
+// batched launches) the compiler cannot tell that they address global
+// memory and would emit flat instructions, which also count against the
+// LDS wait counter; device code accesses them through gmem(), which states
+// the address space, and gatomic_add() (relaxed, device scope, as atomicAdd).
+#define SPIC_GLOBAL __attribute__((address_space(1)))
+
+template <typename P>
+__device__ __forceinline__ SPIC_GLOBAL P *gmem(P *p) {
+  return (SPIC_GLOBAL P *)p;
+}
+
+template <typename P, typename V>
+__device__ __forceinline__ P gatomic_add(P *p, V v) {
+  return __hip_atomic_fetch_add(gmem(p), (P)v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // The particles of range t (a tile's segment, or t == ntiles: the overflow
 // region) as a dense sequence: count, and the slot of the r-th.
@@ -41,8 +67,9 @@ struct RangeView {
   }
 };
 
-__host__ __device__ inline RangeView range_view(const int64_t *seg, const unsigned int *cnt,
-                                                int ntiles, unsigned int ovf_cap, int t) {
+template <typename PS, typename PC>
+__host__ __device__ inline RangeView range_view(PS seg, PC cnt, int ntiles,
+                                                unsigned int ovf_cap, int t) {
   RangeView v;
   v.first = seg[t];
   const int64_t cap = t < ntiles ? seg[t + 1] - seg[t] : (int64_t)ovf_cap;
@@ -112,6 +139,11 @@ struct CountArrays {
   namespace NS {                                                           \
   template <typename T>                                                    \
   hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s);   \
+  template <typename T>                                                    \
+  hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab,           \
+                                       const BatchMap &tiles,              \
+                                       const BatchMap &ovf, int tx, int ty,\
+                                       int tz, int lds_fp64, hipStream_t s);\
   }
 SPIC_DECLARE_BINNED_LAUNCHERS(spic_fast)
 SPIC_DECLARE_BINNED_LAUNCHERS(spic_strict)
@@ -138,6 +170,18 @@ hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int t
                                 int nn, hipStream_t s);
 // Zeroes ns count arrays of n entries each.
 hipError_t zero_counts(const CountArrays &c, hipStream_t s);
+
+// The reduce and the count reset of a batch, from the device-resident
+// argument blocks tab[i] of its instances: per instance, the node blocks
+// nodes.start[i].. of 256 nodes each (reduce) and the counter blocks
+// counts.start[i].. of 256 entries each (reset of every species' cnt_out,
+// 3 * (ntiles + 1) entries); max_ns species per launch row.
+template <typename T>
+hipError_t launch_batch_reduce(const BinnedArgs<T> *tab, const BatchMap &nodes,
+                               int max_ns, int tx, int ty, int tz, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_zero_counts(const BinnedArgs<T> *tab, const BatchMap &counts,
+                                    int max_ns, hipStream_t s);
 
 // Sort of n particles of one species into binned storage (spic_binsort.hip).
 // The source is dense (src_seg == nullptr: particles 0..n-1, ids assigned

@@ -148,9 +148,9 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
             }
       } else {
         ++n_glob;
-        X0 = g.xn[ix - 1]; X1 = g.xn[ix];
-        Y0 = g.yn[iy - 1]; Y1 = g.yn[iy];
-        Z0 = g.zn[iz - 1]; Z1 = g.zn[iz];
+        X0 = gmem(g.xn)[ix - 1]; X1 = gmem(g.xn)[ix];
+        Y0 = gmem(g.yn)[iy - 1]; Y1 = gmem(g.yn)[iy];
+        Z0 = gmem(g.zn)[iz - 1]; Z1 = gmem(g.zn)[iz];
         const int gb = (ix * nyn + iy) * nzn + iz;
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa)
@@ -159,12 +159,12 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
               const int n = gb - aa * sxg - bb * nzn - cc;
-              F[aa][bb][cc][0] = f.Ex[n];
-              F[aa][bb][cc][1] = f.Ey[n];
-              F[aa][bb][cc][2] = f.Ez[n];
-              F[aa][bb][cc][3] = f.Bx[n];
-              F[aa][bb][cc][4] = f.By[n];
-              F[aa][bb][cc][5] = f.Bz[n];
+              F[aa][bb][cc][0] = gmem(f.Ex)[n];
+              F[aa][bb][cc][1] = gmem(f.Ey)[n];
+              F[aa][bb][cc][2] = gmem(f.Ez)[n];
+              F[aa][bb][cc][3] = gmem(f.Bx)[n];
+              F[aa][bb][cc][4] = gmem(f.By)[n];
+              F[aa][bb][cc][5] = gmem(f.Bz)[n];
             }
       }
     }
@@ -257,22 +257,22 @@ template <typename T>
 __device__ __forceinline__ int64_t out_slot(const BinnedArgs<T> &a,
                                             const BinnedSpecies<T> &sp, int dest,
                                             unsigned int slot) {
-  const int64_t sb = sp.seg[dest];
-  if ((int64_t)slot < sp.seg[dest + 1] - sb) return sb + slot;
-  const unsigned int k = atomicAdd(sp.cnt_out + a.ntiles, 1u);
-  return k < sp.ovf_cap ? sp.seg[a.ntiles] + (int64_t)k : -1;
+  const int64_t sb = gmem(sp.seg)[dest];
+  if ((int64_t)slot < gmem(sp.seg)[dest + 1] - sb) return sb + slot;
+  const unsigned int k = gatomic_add(sp.cnt_out + a.ntiles, 1u);
+  return k < sp.ovf_cap ? gmem(sp.seg)[a.ntiles] + (int64_t)k : -1;
 }
 
 template <typename T>
 __device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, int64_t o,
                                                T x, T y, T z, T u, T v, T w, T q,
                                                uint32_t pid) {
-  T *p = sp.out + o;
+  SPIC_GLOBAL T *p = gmem(sp.out) + o;
   const int64_t st = sp.stride;
   p[0] = x; p[st] = y; p[2 * st] = z;
   p[3 * st] = u; p[4 * st] = v; p[5 * st] = w;
   p[6 * st] = q;
-  sp.id_out[o] = pid;
+  gmem(sp.id_out)[o] = pid;
 }
 
 // Write a pushed particle to the output buffer.  A particle that stays in
@@ -290,14 +290,14 @@ __device__ __forceinline__ bool put_particle(const BinnedArgs<T> &a,
   int64_t o = -1;
   if (stay) {
     const int64_t k = (int64_t)stay_base + j;
-    if (k < sp.seg[tile + 1] - sp.seg[tile]) o = sp.seg[tile] + k;
+    if (k < gmem(sp.seg)[tile + 1] - gmem(sp.seg)[tile]) o = gmem(sp.seg)[tile] + k;
   } else {
-    const unsigned int k = atomicAdd(sp.cnt_out + dest, 1u);
-    if ((int64_t)k < sp.seg[dest + 1] - sp.seg[dest]) o = sp.seg[dest] + k;
+    const unsigned int k = gatomic_add(sp.cnt_out + dest, 1u);
+    if ((int64_t)k < gmem(sp.seg)[dest + 1] - gmem(sp.seg)[dest]) o = gmem(sp.seg)[dest] + k;
   }
   if (o < 0) {
-    const unsigned int k = atomicAdd(sp.cnt_out + a.ntiles, 1u);
-    if (k < sp.ovf_cap) o = sp.seg[a.ntiles] + (int64_t)k;
+    const unsigned int k = gatomic_add(sp.cnt_out + a.ntiles, 1u);
+    if (k < sp.ovf_cap) o = gmem(sp.seg)[a.ntiles] + (int64_t)k;
   }
   if (o < 0) return false;
   write_particle(sp, o, x, y, z, u, v, w, q, pid);
@@ -328,9 +328,9 @@ __device__ __forceinline__ void deposit_global(const KGrid<T> &g, const KMom &m,
                                                int iy, int iz, T x, T y, T z, T u, T v,
                                                T w, T q) {
   const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
-  const T xi[2] = {x - g.xn[ix - 1], g.xn[ix] - x};
-  const T eta[2] = {y - g.yn[iy - 1], g.yn[iy] - y};
-  const T zeta[2] = {z - g.zn[iz - 1], g.zn[iz] - z};
+  const T xi[2] = {x - gmem(g.xn)[ix - 1], gmem(g.xn)[ix] - x};
+  const T eta[2] = {y - gmem(g.yn)[iy - 1], gmem(g.yn)[iy] - y};
+  const T zeta[2] = {z - gmem(g.zn)[iz - 1], gmem(g.zn)[iz] - z};
   const T iv = g.invVOL;
   const T uu = u * u, uv = u * v, uw = u * w, vv = v * v, vw = v * w, ww = w * w;
   const int gb = (ix * nyn + iy) * nzn + iz;
@@ -347,7 +347,7 @@ __device__ __forceinline__ void deposit_global(const KGrid<T> &g, const KMom &m,
         add_terms<T, double>(t, wgt, iv, u, v, w, uu, uv, uw, vv, vw, ww);
         const int n = gb - aa * sxg - bb * nzn - cc;
 #pragma unroll
-        for (int mm = 0; mm < 10; ++mm) unsafeAtomicAdd(m.m[mm] + n, t[mm]);
+        for (int mm = 0; mm < 10; ++mm) gatomic_add(m.m[mm] + n, t[mm]);
       }
 }
 
@@ -370,9 +370,9 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
     eta = bb == 0 ? y - s_yn[by] : s_yn[by + 1] - y;
     zeta = cc == 0 ? z - s_zn[bz] : s_zn[bz + 1] - z;
   } else {
-    xi = aa == 0 ? x - g.xn[ix - 1] : g.xn[ix] - x;
-    eta = bb == 0 ? y - g.yn[iy - 1] : g.yn[iy] - y;
-    zeta = cc == 0 ? z - g.zn[iz - 1] : g.zn[iz] - z;
+    xi = aa == 0 ? x - gmem(g.xn)[ix - 1] : gmem(g.xn)[ix] - x;
+    eta = bb == 0 ? y - gmem(g.yn)[iy - 1] : gmem(g.yn)[iy] - y;
+    zeta = cc == 0 ? z - gmem(g.zn)[iz - 1] : gmem(g.zn)[iz] - z;
   }
   const T iv = g.invVOL;
   const T wgt = q * xi * eta * zeta * iv;
@@ -387,7 +387,7 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
   } else {
     const int n = ((ix - aa) * g.nyn + (iy - bb)) * g.nzn + (iz - cc);
 #pragma unroll
-    for (int k = 0; k < 10; ++k) unsafeAtomicAdd(m.m[k] + n, (double)t[k]);
+    for (int k = 0; k < 10; ++k) gatomic_add(m.m[k] + n, (double)t[k]);
   }
   return in_block;
 }
@@ -396,9 +396,7 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
 // per-thread register sums are in the particle precision T.  FP32 kernels
 // ask for three waves per SIMD (at most 168 VGPRs).
 template <typename T, typename A, int TX, int TY, int TZ>
-__global__ __launch_bounds__(4 * TX * TY * TZ)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
-void k_binned_phase(BinnedArgs<T> a) {
+__device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const int tile) {
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
   constexpr int NT = 4 * NC;        // threads: four per cell
@@ -422,7 +420,6 @@ void k_binned_phase(BinnedArgs<T> a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int tile = blockIdx.x;
   const int tz = tile % a.ntz, ty = (tile / a.ntz) % a.nty, tx = tile / (a.ntz * a.nty);
   // block node b along x is global node c0x + b; cell c of the tile uses
   // block nodes c+1 and c+2
@@ -434,12 +431,12 @@ void k_binned_phase(BinnedArgs<T> a) {
 
   // 1. stage node coordinates and fields
   for (int b = tid; b < BX + BY + BZ; b += NT) {
-    if (b < BX) s_xn[b] = (blk.c0x + b < a.g.nxn) ? a.g.xn[blk.c0x + b] : T(0);
+    if (b < BX) s_xn[b] = (blk.c0x + b < a.g.nxn) ? gmem(a.g.xn)[blk.c0x + b] : T(0);
     else if (b < BX + BY)
-      s_yn[b - BX] = (blk.c0y + b - BX < nyn) ? a.g.yn[blk.c0y + b - BX] : T(0);
+      s_yn[b - BX] = (blk.c0y + b - BX < nyn) ? gmem(a.g.yn)[blk.c0y + b - BX] : T(0);
     else
       s_zn[b - BX - BY] =
-          (blk.c0z + b - BX - BY < nzn) ? a.g.zn[blk.c0z + b - BX - BY] : T(0);
+          (blk.c0z + b - BX - BY < nzn) ? gmem(a.g.zn)[blk.c0z + b - BX - BY] : T(0);
   }
   for (int b = tid; b < NB; b += NT) {
     const int bz = b % BZ, by = (b / BZ) % BY, bx = b / (BZ * BY);
@@ -447,8 +444,8 @@ void k_binned_phase(BinnedArgs<T> a) {
     T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (gx < a.g.nxn && gy < nyn && gz < nzn) {
       const int n = (gx * nyn + gy) * nzn + gz;
-      v[0] = a.f.Ex[n]; v[1] = a.f.Ey[n]; v[2] = a.f.Ez[n];
-      v[3] = a.f.Bx[n]; v[4] = a.f.By[n]; v[5] = a.f.Bz[n];
+      v[0] = gmem(a.f.Ex)[n]; v[1] = gmem(a.f.Ey)[n]; v[2] = gmem(a.f.Ez)[n];
+      v[3] = gmem(a.f.Bx)[n]; v[4] = gmem(a.f.By)[n]; v[5] = gmem(a.f.Bz)[n];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) s_fld[b * 6 + c] = v[c];
@@ -465,7 +462,7 @@ void k_binned_phase(BinnedArgs<T> a) {
 
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const RangeView in = range_view(sp.seg, sp.cnt_in, a.ntiles, sp.ovf_cap, tile);
+    const RangeView in = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, tile);
     const int64_t nin = in.count;
     if (nin == 0) continue;  // uniform across the workgroup
     for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
@@ -475,7 +472,7 @@ void k_binned_phase(BinnedArgs<T> a) {
       s_stay = 0u;
       // at most nin particles stay: reserve that many slots of the output
       // segment in one block
-      s_base = atomicAdd(sp.cnt_out + tile, (unsigned int)nin);
+      s_base = gatomic_add(sp.cnt_out + tile, (unsigned int)nin);
     }
     // entries [0, lst_done) of the ring have been deposited; the same value
     // in every thread
@@ -495,11 +492,11 @@ void k_binned_phase(BinnedArgs<T> a) {
       uint32_t pid = 0;
       if (active) {
         const int64_t i = in.slot(r);
-        const T *p = sp.in + i;
+        const SPIC_GLOBAL T *p = gmem(sp.in) + i;
         const int64_t st = sp.stride;
         x = p[0]; y = p[st]; z = p[2 * st];
         u = p[3 * st]; v = p[4 * st]; w = p[5 * st]; q = p[6 * st];
-        pid = sp.id_in[i];
+        pid = gmem(sp.id_in)[i];
       }
       int ix = 2, iy = 2, iz = 2, dest = -1;
       if (active) {
@@ -666,37 +663,64 @@ void k_binned_phase(BinnedArgs<T> a) {
           atomicAdd(&s_mom[m * NB + nb], (A)acc[cc * 10 + m] * scale);
     }
     __syncthreads();
-    double *dst = a.partials + ((size_t)tile * a.ns + s) * (10 * NB);
+    SPIC_GLOBAL double *dst = gmem(a.partials) + ((size_t)tile * a.ns + s) * (10 * NB);
     for (int i = tid; i < 10 * NB; i += NT) dst[i] = (double)s_mom[i];
     if (tid == 0) {
       // the reserved block [stay_base, stay_base + nin) holds s_stay
       // particles; its remainder inside the segment stays unused
-      const int64_t cap = sp.seg[tile + 1] - sp.seg[tile];
+      const int64_t cap = gmem(sp.seg)[tile + 1] - gmem(sp.seg)[tile];
       const int64_t room = cap > (int64_t)stay_base ? cap - (int64_t)stay_base : 0;
       const int64_t lo = (int64_t)stay_base + min((int64_t)s_stay, room);
       const int64_t hi = min((int64_t)stay_base + nin, cap);
       if (hi > lo) {
-        sp.cnt_out[(a.ntiles + 1) + tile] = (unsigned int)lo;
-        sp.cnt_out[2 * (a.ntiles + 1) + tile] = (unsigned int)(hi - lo);
+        gmem(sp.cnt_out)[(a.ntiles + 1) + tile] = (unsigned int)lo;
+        gmem(sp.cnt_out)[2 * (a.ntiles + 1) + tile] = (unsigned int)(hi - lo);
       }
     }
     __syncthreads();
   }
-  if (bad_move) atomicAdd(a.status, 1u);
-  if (bad_dep) atomicAdd(a.status + 1, 1u);
-  if (bad_store) atomicAdd(a.status + 2, 1u);
+  if (bad_move) gatomic_add(a.status, 1u);
+  if (bad_dep) gatomic_add(a.status + 1, 1u);
+  if (bad_store) gatomic_add(a.status + 2, 1u);
   if (a.counters) {
-    if (n_gg) atomicAdd(a.counters + 0, (unsigned long long)n_gg);
-    if (n_gd) atomicAdd(a.counters + 1, (unsigned long long)n_gd);
-    if (n_lg) atomicAdd(a.counters + 2, (unsigned long long)n_lg);
-    if (n_ld) atomicAdd(a.counters + 3, (unsigned long long)n_ld);
+    if (n_gg) gatomic_add(a.counters + 0, (unsigned long long)n_gg);
+    if (n_gd) gatomic_add(a.counters + 1, (unsigned long long)n_gd);
+    if (n_lg) gatomic_add(a.counters + 2, (unsigned long long)n_lg);
+    if (n_ld) gatomic_add(a.counters + 3, (unsigned long long)n_ld);
   }
+}
+
+// Instance of block b in a batch map (uniform across the workgroup).
+__device__ __forceinline__ int batch_instance(const BatchMap &map, int b) {
+  int i = 0;
+  for (int k = 1; k < map.n; ++k) i += b >= map.start[k] ? 1 : 0;
+  return i;
+}
+
+template <typename T, typename A, int TX, int TY, int TZ>
+__global__ __launch_bounds__(4 * TX * TY * TZ)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+void k_binned_phase(BinnedArgs<T> a) {
+  binned_phase_tile<T, A, TX, TY, TZ>(a, blockIdx.x);
+}
+
+// Every instance of a batch in one grid: instance i owns blocks
+// [map.start[i], map.start[i + 1]), one per tile, with arguments tab[i].
+template <typename T, typename A, int TX, int TY, int TZ>
+__global__ __launch_bounds__(4 * TX * TY * TZ)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+void k_binned_phase_batch(const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
+  const int i = batch_instance(map, blockIdx.x);
+  binned_phase_tile<T, A, TX, TY, TZ>(tab[i], blockIdx.x - map.start[i]);
 }
 
 // The particles of every species' input overflow region: global gathers
 // and deposits, re-binned like the tile kernel's.
+// Part `part` of `nparts` (workgroups of 256 threads) takes every
+// (nparts * 256)-th particle.
 template <typename T, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
+__device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int part,
+                                                     int nparts) {
   Block<T, 1, 1, 1> none;
   none.fld = nullptr;
   none.xn = none.yn = none.zn = nullptr;
@@ -705,17 +729,17 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
   unsigned int n_gg = 0, n_gd = 0, n_lg = 0;
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const RangeView ov = range_view(sp.seg, sp.cnt_in, a.ntiles, sp.ovf_cap, a.ntiles);
+    const RangeView ov = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, a.ntiles);
     const int64_t nin = ov.count;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nin;
-         r += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t r = part * (int64_t)256 + threadIdx.x; r < nin;
+         r += (int64_t)nparts * 256) {
       const int64_t i = ov.slot(r);
-      const T *p = sp.in + i;
+      const SPIC_GLOBAL T *p = gmem(sp.in) + i;
       const int64_t st = sp.stride;
       T x = p[0], y = p[st], z = p[2 * st];
       T u = p[3 * st], v = p[4 * st], w = p[5 * st];
       const T q = p[6 * st];
-      const uint32_t pid = sp.id_in[i];
+      const uint32_t pid = gmem(sp.id_in)[i];
       push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
                                        bad_move, n_lg, n_gg);
       int ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
@@ -725,20 +749,33 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
       bad_dep |= bclamp(iy, a.g.nyn);
       bad_dep |= bclamp(iz, a.g.nzn);
       const int dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
-      const int64_t o = out_slot(a, sp, dest, atomicAdd(sp.cnt_out + dest, 1u));
+      const int64_t o = out_slot(a, sp, dest, gatomic_add(sp.cnt_out + dest, 1u));
       if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
       else bad_store = true;
       ++n_gd;
       deposit_global<T>(a.g, sp.m, ix, iy, iz, x, y, z, u, v, w, q);
     }
   }
-  if (bad_move) atomicAdd(a.status, 1u);
-  if (bad_dep) atomicAdd(a.status + 1, 1u);
-  if (bad_store) atomicAdd(a.status + 2, 1u);
+  if (bad_move) gatomic_add(a.status, 1u);
+  if (bad_dep) gatomic_add(a.status + 1, 1u);
+  if (bad_store) gatomic_add(a.status + 2, 1u);
   if (a.counters) {
-    if (n_gg) atomicAdd(a.counters + 0, (unsigned long long)n_gg);
-    if (n_gd) atomicAdd(a.counters + 1, (unsigned long long)n_gd);
+    if (n_gg) gatomic_add(a.counters + 0, (unsigned long long)n_gg);
+    if (n_gd) gatomic_add(a.counters + 1, (unsigned long long)n_gd);
   }
+}
+
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
+  binned_overflow_part<T, TX, TY, TZ>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_binned_overflow_batch(
+    const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
+  const int i = batch_instance(map, blockIdx.x);
+  binned_overflow_part<T, TX, TY, TZ>(tab[i], blockIdx.x - map.start[i],
+                                      map.start[i + 1] - map.start[i]);
 }
 
 template <typename T, typename A>
@@ -768,6 +805,38 @@ hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
 
 template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);
 template hipError_t launch_binned_phase<float>(const BinnedArgs<float> &, hipStream_t);
+
+template <typename T, typename A, int TX, int TY, int TZ>
+hipError_t launch_batch_shape(const BinnedArgs<T> *tab, const BatchMap &tiles,
+                              const BatchMap &ovf, hipStream_t s) {
+  hipLaunchKernelGGL((k_binned_phase_batch<T, A, TX, TY, TZ>), dim3(tiles.start[tiles.n]),
+                     dim3(4 * TX * TY * TZ), 0, s, tab, tiles);
+  hipLaunchKernelGGL((k_binned_overflow_batch<T, TX, TY, TZ>), dim3(ovf.start[ovf.n]),
+                     dim3(256), 0, s, tab, ovf);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab, const BatchMap &tiles,
+                                     const BatchMap &ovf, int tx, int ty, int tz,
+                                     int lds_fp64, hipStream_t s) {
+  if (tiles.n <= 0 || tiles.start[tiles.n] <= 0) return hipSuccess;
+  const bool wide = sizeof(T) == 4 && lds_fp64;
+  if (tx == 4 && ty == 4 && tz == 4)
+    return wide ? launch_batch_shape<T, double, 4, 4, 4>(tab, tiles, ovf, s)
+                : launch_batch_shape<T, T, 4, 4, 4>(tab, tiles, ovf, s);
+  if (tx == 8 && ty == 4 && tz == 4)
+    return wide ? launch_batch_shape<T, double, 8, 4, 4>(tab, tiles, ovf, s)
+                : launch_batch_shape<T, T, 8, 4, 4>(tab, tiles, ovf, s);
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_binned_phase_batch<double>(const BinnedArgs<double> *,
+                                                      const BatchMap &, const BatchMap &,
+                                                      int, int, int, int, hipStream_t);
+template hipError_t launch_binned_phase_batch<float>(const BinnedArgs<float> *,
+                                                     const BatchMap &, const BatchMap &,
+                                                     int, int, int, int, hipStream_t);
 
 }  // namespace SPIC_NS
 }  // namespace spic

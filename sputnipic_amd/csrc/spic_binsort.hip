@@ -174,21 +174,19 @@ __global__ void k_zero_counts(CountArrays c) {
 // Tiles whose input segment held no particle of the species wrote no
 // partials and are skipped.
 template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int nn) {
+__device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int ntx, int nty,
+                                            int ntz, int ntiles, int ns,
+                                            const double *__restrict__ partials,
+                                            const int64_t *seg, const unsigned int *cnt,
+                                            double *const *mom) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
-  const int s = blockIdx.y;
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= nn) return;
-  const int nyn = a.nyn, nzn = a.nzn;
   const int gz = n % nzn, gy = (n / nzn) % nyn, gx = n / (nzn * nyn);
   const int xl = gx >= BX - 1 ? (gx - (BX - 1) + TX - 1) / TX : 0;
-  const int xh = min(gx / TX, a.ntx - 1);
+  const int xh = min(gx / TX, ntx - 1);
   const int yl = gy >= BY - 1 ? (gy - (BY - 1) + TY - 1) / TY : 0;
-  const int yh = min(gy / TY, a.nty - 1);
+  const int yh = min(gy / TY, nty - 1);
   const int zl = gz >= BZ - 1 ? (gz - (BZ - 1) + TZ - 1) / TZ : 0;
-  const int zh = min(gz / TZ, a.ntz - 1);
-  const unsigned int *cnt = a.cnt[s];
-  const int64_t *seg = a.seg[s];
+  const int zh = min(gz / TZ, ntz - 1);
   double sum[10], cmp[10];
 #pragma unroll
   for (int m = 0; m < 10; ++m) { sum[m] = 0.0; cmp[m] = 0.0; }
@@ -196,11 +194,11 @@ __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int n
   for (int tx = xl; tx <= xh; ++tx)
     for (int ty = yl; ty <= yh; ++ty)
       for (int tz = zl; tz <= zh; ++tz) {
-        const int tile = (tx * a.nty + ty) * a.ntz + tz;
-        if (range_view(seg, cnt, a.ntiles, 0u, tile).count == 0) continue;
+        const int tile = (tx * nty + ty) * ntz + tz;
+        if (range_view(gmem(seg), gmem(cnt), ntiles, 0u, tile).count == 0) continue;
         any = true;
         const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
-        const double *src = a.partials + ((size_t)tile * a.ns + s) * (10 * NB) + b;
+        const SPIC_GLOBAL double *src = gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + b;
 #pragma unroll
         for (int m = 0; m < 10; ++m) {
           const double v = src[m * NB];
@@ -211,7 +209,42 @@ __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int n
       }
   if (!any) return;
 #pragma unroll
-  for (int m = 0; m < 10; ++m) a.mom[s].m[m][n] += sum[m] + cmp[m];
+  for (int m = 0; m < 10; ++m) gmem(mom[m])[n] += sum[m] + cmp[m];
+}
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int nn) {
+  const int s = blockIdx.y;
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= nn) return;
+  reduce_node<TX, TY, TZ>(n, s, a.nyn, a.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
+                          a.partials, a.seg[s], a.cnt[s], a.mom[s].m);
+}
+
+// The reduce of a batch: blockIdx.y is the species.
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_binned_reduce_batch(
+    const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
+  int i = 0;
+  for (int k = 1; k < map.n; ++k) i += (int)blockIdx.x >= map.start[k] ? 1 : 0;
+  const BinnedArgs<T> &a = tab[i];
+  const int s = blockIdx.y;
+  const int n = (blockIdx.x - map.start[i]) * 256 + threadIdx.x;
+  if (s >= a.ns || n >= a.g.nxn * a.g.nyn * a.g.nzn) return;
+  reduce_node<TX, TY, TZ>(n, s, a.g.nyn, a.g.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
+                          a.partials, a.sp[s].seg, a.sp[s].cnt_in, a.sp[s].m.m);
+}
+
+// The count reset of a batch: blockIdx.y is the species.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_counts_batch(
+    const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
+  int i = 0;
+  for (int k = 1; k < map.n; ++k) i += (int)blockIdx.x >= map.start[k] ? 1 : 0;
+  const BinnedArgs<T> &a = tab[i];
+  const int s = blockIdx.y;
+  const int j = (blockIdx.x - map.start[i]) * 256 + threadIdx.x;
+  if (s < a.ns && j < 3 * (a.ntiles + 1)) gmem(a.sp[s].cnt_out)[j] = 0u;
 }
 
 }  // namespace
@@ -228,6 +261,41 @@ hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int t
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_batch_reduce(const BinnedArgs<T> *tab, const BatchMap &nodes,
+                               int max_ns, int tx, int ty, int tz, hipStream_t s) {
+  if (nodes.n <= 0 || nodes.start[nodes.n] <= 0 || max_ns <= 0) return hipSuccess;
+  dim3 grid(nodes.start[nodes.n], max_ns);
+  if (tx == 4 && ty == 4 && tz == 4)
+    hipLaunchKernelGGL((k_binned_reduce_batch<T, 4, 4, 4>), grid, dim3(256), 0, s, tab,
+                       nodes);
+  else if (tx == 8 && ty == 4 && tz == 4)
+    hipLaunchKernelGGL((k_binned_reduce_batch<T, 8, 4, 4>), grid, dim3(256), 0, s, tab,
+                       nodes);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_zero_counts(const BinnedArgs<T> *tab, const BatchMap &counts,
+                                    int max_ns, hipStream_t s) {
+  if (counts.n <= 0 || counts.start[counts.n] <= 0 || max_ns <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_zero_counts_batch<T>), dim3(counts.start[counts.n], max_ns),
+                     dim3(256), 0, s, tab, counts);
+  return hipGetLastError();
+}
+
+template hipError_t launch_batch_reduce<double>(const BinnedArgs<double> *,
+                                                const BatchMap &, int, int, int, int,
+                                                hipStream_t);
+template hipError_t launch_batch_reduce<float>(const BinnedArgs<float> *, const BatchMap &,
+                                               int, int, int, int, hipStream_t);
+template hipError_t launch_batch_zero_counts<double>(const BinnedArgs<double> *,
+                                                     const BatchMap &, int, hipStream_t);
+template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
+                                                    const BatchMap &, int, hipStream_t);
 
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;

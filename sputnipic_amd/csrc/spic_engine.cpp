@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "spic.h"
 #include "spic_binned.hpp"
@@ -24,6 +25,7 @@ struct spic_engine {
   int sort_every = 0;
   int lds_fp64 = 1;
   int64_t phases = 0, sorts = 0;
+  uint64_t gen = 0;                // bumped by every load (storage may move)
   size_t bytes = 0;
   struct Species {
     bool loaded = false;
@@ -131,11 +133,13 @@ hipError_t sort_any(spic_engine *e, int is, void *const *src, hipStream_t s) {
              : sort_species<float>(e, is, (float *const *)src, s);
 }
 
+// The phase kernel's argument block for the loaded species of `e`, read
+// from buffer `cur` in layout `segi` (the species' own when negative).
+// idx[k] is the engine species of argument species k; returns the count.
 template <typename T>
-hipError_t launch_phase(spic_engine *e, const spic_field *field,
-                        const spic_moments *mom, const spic_push *push,
-                        unsigned int *status, hipStream_t s) {
-  spic::BinnedArgs<T> a;
+int fill_args(const spic_engine *e, const spic_field *field, const spic_moments *mom,
+              const spic_push *push, unsigned int *status, int cur, int segi,
+              spic::BinnedArgs<T> &a, int *idx) {
   std::memset(&a, 0, sizeof(a));
   a.g = spic::kgrid<T>(&e->grd);
   a.f = spic::kfield<T>(field);
@@ -148,24 +152,19 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   a.status = status;
   a.counters = e->counters;
   a.partials = e->partials;
-  spic::BinnedReduceArgs r;
-  std::memset(&r, 0, sizeof(r));
-  spic::CountArrays zc;
-  std::memset(&zc, 0, sizeof(zc));
   int k = 0;
-  int idx[spic::kMaxSpecies];
   for (int is = 0; is < e->ns; ++is) {
-    auto &sp = e->sp[is];
+    const auto &sp = e->sp[is];
     if (!sp.loaded || sp.n == 0) continue;
-    idx[k] = is;
+    if (idx) idx[k] = is;
     auto &t = a.sp[k];
-    const int in = sp.cur, out = 1 - sp.cur;
+    const int in = cur < 0 ? sp.cur : cur, out = 1 - in;
     t.in = (const T *)sp.block[in];
     t.out = (T *)sp.block[out];
     t.stride = sp.cap;
     t.id_in = sp.id[in];
     t.id_out = sp.id[out];
-    t.seg = sp.seg[sp.segi];
+    t.seg = sp.seg[segi < 0 ? sp.segi : segi];
     t.cnt_in = sp.cnt[in];
     t.cnt_out = sp.cnt[out];
     t.ovf_cap = sp.ovf_cap;
@@ -176,14 +175,31 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     desc.n_sub_cycles = sp.nsub;
     t.k = spic::kpush<T>(&desc, push);
     t.m = spic::kmom(&mom[is]);
-    r.cnt[k] = sp.cnt[in];
-    r.seg[k] = sp.seg[sp.segi];
-    r.mom[k] = t.m;
-    zc.p[k] = sp.cnt[out];
     ++k;
   }
+  a.ns = k;
+  return k;
+}
+
+template <typename T>
+hipError_t launch_phase(spic_engine *e, const spic_field *field,
+                        const spic_moments *mom, const spic_push *push,
+                        unsigned int *status, hipStream_t s) {
+  spic::BinnedArgs<T> a;
+  int idx[spic::kMaxSpecies];
+  const int k = fill_args<T>(e, field, mom, push, status, -1, -1, a, idx);
   if (k == 0) return hipSuccess;
-  a.ns = r.ns = zc.ns = k;
+  spic::BinnedReduceArgs r;
+  std::memset(&r, 0, sizeof(r));
+  spic::CountArrays zc;
+  std::memset(&zc, 0, sizeof(zc));
+  for (int j = 0; j < k; ++j) {
+    r.cnt[j] = a.sp[j].cnt_in;
+    r.seg[j] = a.sp[j].seg;
+    r.mom[j] = a.sp[j].m;
+    zc.p[j] = a.sp[j].cnt_out;
+  }
+  r.ns = zc.ns = k;
   zc.n = 3 * (e->L.ntiles + 1);
   hipError_t err = spic::zero_counts(zc, s);
   if (err != hipSuccess) return err;
@@ -349,6 +365,7 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
   sp.cur = 0;
   sp.segi = 0;
   sp.loaded = true;
+  e->gen++;
   void *srcs[7] = {src->x, src->y, src->z, src->u, src->v, src->w, src->q};
   if ((err = sort_any(e, is, srcs, s)) != hipSuccess) {
     sp.loaded = false;
@@ -455,6 +472,223 @@ int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
     if (err != hipSuccess) return hip_fail(err, "engine overflow count");
     st->overflow += ovf < sp.ovf_cap ? ovf : sp.ovf_cap;
   }
+  return SPIC_OK;
+}
+
+}  // extern "C"
+
+// ---- batches --------------------------------------------------------------
+
+struct spic_batch {
+  int n = 0;
+  int prec = SPIC_F64;
+  int sort_every = 0;
+  int64_t phases = 0;
+  spic_engine *e[spic::kMaxInstances] = {};
+  spic_field field[spic::kMaxInstances];
+  spic_moments mom[spic::kMaxInstances][spic::kMaxSpecies];
+  spic_push push[spic::kMaxInstances];
+  // the argument tables were built for these engine generations and this
+  // status buffer
+  uint64_t gen[spic::kMaxInstances] = {};
+  unsigned int *status = nullptr;
+  // device: four tables of n argument blocks, one per (layout, buffer)
+  // state: table (segi * 2 + cur)
+  void *tab = nullptr;
+  size_t arg_bytes = 0;
+  spic::BatchMap tiles{}, ovf{}, nodes{}, counts{};
+  int max_ns = 0;
+};
+
+namespace {
+
+constexpr int kBatchOverflowBlocks = 64;  // overflow workgroups per instance
+
+template <typename T>
+hipError_t build_tables(spic_batch *b, unsigned int *status) {
+  const size_t nb = sizeof(spic::BinnedArgs<T>);
+  std::vector<spic::BinnedArgs<T>> host(4 * (size_t)b->n);
+  b->max_ns = 0;
+  for (int st = 0; st < 4; ++st)
+    for (int i = 0; i < b->n; ++i) {
+      const int k = fill_args<T>(b->e[i], &b->field[i], b->mom[i], &b->push[i], status,
+                                 st & 1, st >> 1, host[(size_t)st * b->n + i], nullptr);
+      if (k > b->max_ns) b->max_ns = k;
+    }
+  hipError_t err;
+  if (!b->tab) {
+    if ((err = hipMalloc(&b->tab, 4 * nb * b->n)) != hipSuccess) return err;
+  }
+  if ((err = hipMemcpy(b->tab, host.data(), 4 * nb * b->n, hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return err;
+  b->arg_bytes = nb;
+  auto blocks = [](int64_t items, int per) { return (int)((items + per - 1) / per); };
+  b->tiles.n = b->ovf.n = b->nodes.n = b->counts.n = b->n;
+  b->tiles.start[0] = b->ovf.start[0] = b->nodes.start[0] = b->counts.start[0] = 0;
+  for (int i = 0; i < b->n; ++i) {
+    const spic_engine *e = b->e[i];
+    const bool any = host[i].ns > 0;
+    const int64_t nn = (int64_t)e->grd.nxn * e->grd.nyn * e->grd.nzn;
+    b->tiles.start[i + 1] = b->tiles.start[i] + (any ? e->L.ntiles : 0);
+    b->ovf.start[i + 1] = b->ovf.start[i] + (any ? kBatchOverflowBlocks : 0);
+    b->nodes.start[i + 1] = b->nodes.start[i] + (any ? blocks(nn, 256) : 0);
+    b->counts.start[i + 1] =
+        b->counts.start[i] + (any ? blocks(3 * (int64_t)(e->L.ntiles + 1), 256) : 0);
+    b->gen[i] = e->gen;
+  }
+  b->status = status;
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
+  const spic::BinnedArgs<T> *tab =
+      (const spic::BinnedArgs<T> *)((char *)b->tab + (size_t)state * b->n * b->arg_bytes);
+  const spic::BinLayout &L = b->e[0]->L;
+  hipError_t err = spic::launch_batch_zero_counts<T>(tab, b->counts, b->max_ns, s);
+  if (err != hipSuccess) return err;
+  err = spic::strict_mode()
+            ? spic::spic_strict::launch_binned_phase_batch<T>(
+                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, s)
+            : spic::spic_fast::launch_binned_phase_batch<T>(
+                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, s);
+  if (err != hipSuccess) return err;
+  return spic::launch_batch_reduce<T>(tab, b->nodes, b->max_ns, L.tx, L.ty, L.tz, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
+                      int sort_every) {
+  if (!out) return fail(SPIC_EINVAL, "batch pointer is NULL");
+  *out = nullptr;
+  if (!members) return fail(SPIC_EINVAL, "batch members are NULL");
+  if (n < 1 || n > spic::kMaxInstances)
+    return fail(SPIC_EINVAL, "batch instance count must be 1..%d", spic::kMaxInstances);
+  if (sort_every < 0) return fail(SPIC_EINVAL, "sort_every must be >= 0");
+  int rc;
+  for (int i = 0; i < n; ++i) {
+    const spic_batch_member &m = members[i];
+    if (!m.engine) return fail(SPIC_EINVAL, "batch member %d: engine is NULL", i);
+    if (!m.field || !m.push || !m.moments)
+      return fail(SPIC_EINVAL, "batch member %d: field, moments or push is NULL", i);
+    if ((rc = spic::check_field(m.field)) || (rc = spic::check_push(m.push))) return rc;
+    const spic_engine *e = m.engine, *e0 = members[0].engine;
+    if (e->grd.prec != e0->grd.prec || e->L.tx != e0->L.tx || e->L.ty != e0->L.ty ||
+        e->L.tz != e0->L.tz || e->lds_fp64 != e0->lds_fp64)
+      return fail(SPIC_EINVAL,
+                  "batch member %d: precision, tile shape or LDS partials differ from "
+                  "member 0's", i);
+    for (int j = 0; j < i; ++j)
+      if (members[j].engine == e)
+        return fail(SPIC_EINVAL, "batch members %d and %d share an engine", j, i);
+    for (int is = 0; is < e->ns; ++is)
+      if (e->sp[is].loaded && (rc = spic::check_moments(&m.moments[is]))) return rc;
+  }
+  spic_batch *b = new (std::nothrow) spic_batch();
+  if (!b) return fail(SPIC_ENOMEM, "batch allocation");
+  b->n = n;
+  b->prec = members[0].engine->grd.prec;
+  b->sort_every = sort_every;
+  for (int i = 0; i < n; ++i) {
+    b->e[i] = members[i].engine;
+    b->field[i] = *members[i].field;
+    b->push[i] = *members[i].push;
+    std::memset(b->mom[i], 0, sizeof(b->mom[i]));
+    for (int is = 0; is < b->e[i]->ns; ++is)
+      if (b->e[i]->sp[is].loaded) b->mom[i][is] = members[i].moments[is];
+  }
+  hipError_t err;
+  unsigned int *st = spic::status_buffer(&err);
+  if (!st) {
+    delete b;
+    return hip_fail(err, "status buffer");
+  }
+  err = b->prec == SPIC_F64 ? build_tables<double>(b, st) : build_tables<float>(b, st);
+  if (err != hipSuccess) {
+    spic_batch_destroy(b);
+    return hip_fail(err, "batch argument tables");
+  }
+  *out = b;
+  return SPIC_OK;
+}
+
+int spic_batch_phase(spic_batch *b, void *stream) {
+  if (!b) return fail(SPIC_EINVAL, "batch is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t err;
+  // every loaded species of the batch must be in one (layout, buffer) state
+  int state = -1;
+  bool stale = false;
+  for (int i = 0; i < b->n; ++i) {
+    const spic_engine *e = b->e[i];
+    if (e->gen != b->gen[i]) stale = true;
+    for (int is = 0; is < e->ns; ++is) {
+      const auto &sp = e->sp[is];
+      if (!sp.loaded || sp.n == 0) continue;
+      if (!b->mom[i][is].rhon)
+        return fail(SPIC_EINVAL, "batch member %d: species %d loaded after the batch "
+                                 "was created without moments", i, is);
+      const int st = sp.segi * 2 + sp.cur;
+      if (state < 0) state = st;
+      else if (st != state)
+        return fail(SPIC_EINVAL,
+                    "batch species out of step (member %d species %d): load every "
+                    "species, then advance them only through the batch", i, is);
+    }
+  }
+  if (state < 0) return SPIC_OK;  // nothing loaded
+  const bool sort_now = b->sort_every > 0 && b->phases > 0 && b->phases % b->sort_every == 0;
+  unsigned int *st;
+  int rc;
+  if ((rc = spic::begin_call(s, &st))) return rc;
+  if (stale || st != b->status) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if ((err = hipStreamIsCapturing(s, &cs)) != hipSuccess)
+      return hip_fail(err, "batch capture status");
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(SPIC_EINVAL, "batch argument tables are stale (a member was loaded, or "
+                               "another thread calls); run one phase outside capture");
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(err, "batch phase");
+    err = b->prec == SPIC_F64 ? build_tables<double>(b, st) : build_tables<float>(b, st);
+    if (err != hipSuccess) return hip_fail(err, "batch argument tables");
+  }
+  if (sort_now) {
+    for (int i = 0; i < b->n; ++i) {
+      spic_engine *e = b->e[i];
+      std::lock_guard<std::mutex> lock(e->mu);
+      for (int is = 0; is < e->ns; ++is) {
+        if (!e->sp[is].loaded || e->sp[is].n == 0) continue;
+        if ((err = sort_any(e, is, nullptr, s)) != hipSuccess)
+          return hip_fail(err, "batch sort");
+      }
+    }
+    state ^= 3;  // a sort moves to the other buffer and the other layout
+  }
+  err = b->prec == SPIC_F64 ? launch_batch<double>(b, state, s)
+                            : launch_batch<float>(b, state, s);
+  if (err == hipSuccess) {
+    for (int i = 0; i < b->n; ++i) {
+      spic_engine *e = b->e[i];
+      for (int is = 0; is < e->ns; ++is)
+        if (e->sp[is].loaded && e->sp[is].n > 0) e->sp[is].cur ^= 1;
+      e->phases++;
+    }
+    b->phases++;
+  }
+  return spic::end_call(s, st, err);
+}
+
+int spic_batch_destroy(spic_batch *b) {
+  if (!b) return SPIC_OK;
+  if (b->tab) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->tab);
+  }
+  delete b;
   return SPIC_OK;
 }
 

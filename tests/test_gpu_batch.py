@@ -1,0 +1,166 @@
+"""GPU parity of batches (spic_batch_* through the C ABI): independent
+Harris-sheet instances with different grids, species counts, particle
+counts and perturbation amplitudes, advanced together with one launch per
+kernel and phase.
+
+Each instance's results are the oracle's for that instance run alone.
+Tolerances, as for the engine on its own: in strict arithmetic the particle
+state is bit-identical after every phase, whatever the sort schedule; node
+moments differ only in summation order: 1e-13 of each array's max magnitude
+in FP64, 1e-5 in FP32 (the FP32-form bound).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sputnipic_amd as S
+from helpers import PRECS, assert_bitwise, assert_close
+
+pytestmark = pytest.mark.gpu
+
+# grids with partial tiles, 1..4 species, 1..12 particles per cell
+INSTANCES = [
+    dict(nxc=8, nyc=6, nzc=4, ppc=(2, 2, 2), ns=2, pertX=0.01),
+    dict(nxc=12, nyc=8, nzc=8, ppc=(1, 2, 2), ns=4, pertX=0.05),
+    dict(nxc=16, nyc=4, nzc=10, ppc=(3, 2, 2), ns=3, pertX=0.4),
+    dict(nxc=4, nyc=4, nzc=4, ppc=(1, 1, 1), ns=1, pertX=0.005),
+]
+
+
+@pytest.fixture(autouse=True)
+def _modes():
+    S.set_async(False)
+    S.set_strict_fp(True)
+    yield
+    S.set_async(False)
+    S.set_strict_fp(False)
+
+
+def _oracle(spec, prec):
+    p = spec["ppc"]
+    cfg = O.GemConfig(nxc=spec["nxc"], nyc=spec["nyc"], nzc=spec["nzc"], ns=spec["ns"],
+                      npcelx=[p[0]] * 4, npcely=[p[1]] * 4, npcelz=[p[2]] * 4,
+                      pertX=spec["pertX"])
+    return O.State(cfg, prec)
+
+
+class Instance:
+    def __init__(self, st, prec, tile=(4, 4, 4), lds_partials=0):
+        c = st.cfg
+        dt = np.float64 if prec == "f64" else np.float32
+        self.grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
+        self.field = S.EMField(self.grd, {n: st.fields[n].astype(dt) for n in O.FIELD_NAMES})
+        self.parts = [S.Particles(p, c.qom[s], prec, c.NiterMover)
+                      for s, p in enumerate(st.parts)]
+        self.ids = [S.InterpDensSpecies(self.grd) for _ in range(c.ns)]
+        self.param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover,
+                                  PERIODICX=c.PERIODICX, PERIODICY=c.PERIODICY,
+                                  PERIODICZ=c.PERIODICZ)
+        self.eng = S.Engine(self.grd, c.ns, tile=tile, lds_partials=lds_partials)
+        for s, p in enumerate(self.parts):
+            self.eng.load(s, p)
+        self.ns = c.ns
+
+    def member(self):
+        return (self.eng, self.field, self.ids, self.param)
+
+    def zero(self, stream=None):
+        S.setZeroSpeciesDensities(self.ids, self.grd, self.ns, stream)
+
+    def results(self, stream=None):
+        for s, p in enumerate(self.parts):
+            self.eng.store(s, p, stream)
+        return [p.download() for p in self.parts], [m.download() for m in self.ids]
+
+
+def _check(inst_results, states, prec, what):
+    tol = 1e-13 if prec == "f64" else 1e-5
+    for i, ((got_p, got_m), st) in enumerate(zip(inst_results, states)):
+        for s in range(st.cfg.ns):
+            for k in O.PARTICLE_NAMES:
+                assert_bitwise(got_p[s][k], st.parts[s][k], f"{what} inst {i} s{s} {k}")
+            for n in O.MOMENT_NAMES:
+                assert_close(got_m[s][n], st.moments[s][n], tol, f"{what} inst {i} s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("tile,sort_every", [((4, 4, 4), 0), ((4, 4, 4), 2), ((8, 4, 4), 1)])
+def test_batch_matches_oracle_per_instance(prec, tile, sort_every):
+    states = [_oracle(spec, prec) for spec in INSTANCES]
+    insts = [Instance(st, prec, tile) for st in states]
+    batch = S.Batch([x.member() for x in insts], sort_every=sort_every)
+    phases = 3
+    for _ in range(phases):
+        for x in insts:
+            x.zero()
+        batch.phase()
+    for st in states:
+        for _ in range(phases):
+            st.cycle()
+    _check([x.results() for x in insts], states, prec, f"{tile} sort {sort_every}")
+    for x in insts:
+        stats = x.eng.stats()
+        assert stats["phases"] == phases
+        if sort_every:
+            assert stats["sorts"] == x.ns * (1 + (phases - 1) // sort_every)
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
+def test_batch_graph_replays_match_oracle():
+    """Two phases (zero, batch phase) captured in one graph and replayed
+    twice, each replay checked against the oracle after it."""
+    states = [_oracle(spec, "f64") for spec in INSTANCES]
+    insts = [Instance(st, "f64") for st in states]
+    batch = S.Batch([x.member() for x in insts], sort_every=3)
+    stream = S.Stream()
+    with S.Graph(stream) as g:
+        for _ in range(2):
+            for x in insts:
+                x.zero(stream)
+            batch.phase(stream)
+    for rep in range(2):
+        g.launch()
+        stream.synchronize()
+        S.set_async(False)
+        S.check_domain(stream)
+        for st in states:
+            for _ in range(2):
+                st.cycle()
+        _check([x.results(stream) for x in insts], states, "f64", f"replay {rep}")
+        S.set_async(True)
+    S.set_async(False)
+    g.close()
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
+def test_batch_rejects_bad_members():
+    st64 = _oracle(INSTANCES[0], "f64")
+    st32 = _oracle(INSTANCES[3], "f32")
+    a, b = Instance(st64, "f64"), Instance(st32, "f32")
+    with pytest.raises(S.SpicError) as ei:
+        S.Batch([a.member(), b.member()])
+    assert ei.value.code == S.SPIC_EINVAL
+    with pytest.raises(S.SpicError) as ei:
+        S.Batch([a.member(), a.member()])
+    assert ei.value.code == S.SPIC_EINVAL
+    c = Instance(_oracle(INSTANCES[1], "f64"), "f64", tile=(8, 4, 4))
+    with pytest.raises(S.SpicError) as ei:
+        S.Batch([a.member(), c.member()])
+    assert ei.value.code == S.SPIC_EINVAL
+    # a member advanced on its own falls out of step with the others
+    d = Instance(_oracle(INSTANCES[2], "f64"), "f64")
+    batch = S.Batch([a.member(), d.member()])
+    d.zero()
+    d.eng.phase(d.field, d.ids, d.param)
+    with pytest.raises(S.SpicError) as ei:
+        batch.phase()
+    assert ei.value.code == S.SPIC_EINVAL
+    batch.close()
+    for x in (a, b, c, d):
+        x.eng.close()
