@@ -7,8 +7,9 @@ published run's shape -- 128x64x64 cells on a 20x10x10 box, dt 0.25,
 NiterMover 3, four species (qom -64, 1, -64, 1) at 5x5x5 particles per cell
 = 65,536,000 per species, 262,144,000 in total, all resident in HBM.  The
 fields are frozen at the initial condition (no field solve between cycles,
-which is not on this path).  A step is one cycle: zero the node moments,
-then push and deposit every species.
+which is not on this path).  A step is one cycle: push and deposit every
+species into freshly zeroed node moments (the engine writes the moments,
+which folds the reference driver's zeroing into its reduce).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints ONE JSON line on stdout; progress goes to stderr.
@@ -84,7 +85,8 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
     m_arr = (spic_moments * NS)(*m_c)
     eng = None
     if engine == "tiled":
-        eng = S.Engine(grd, NS, tile=tile, sort_every=0)
+        # the phase writes the moments, so a cycle needs no zeroing pass
+        eng = S.Engine(grd, NS, tile=tile, sort_every=0, set_moments=True)
         t0 = time.perf_counter()
         for s, p in enumerate(parts):
             eng.load(s, p, stream)
@@ -110,8 +112,10 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         return pool[i % len(pool)], pool[(i + 1) % len(pool)]
 
     def cycle(timed: bool):
-        for s in range(NS):
-            check(L.spic_zero_moments(C.byref(g_c), C.byref(m_c[s]), stream.ptr), "zero")
+        if eng is None:
+            for s in range(NS):
+                check(L.spic_zero_moments(C.byref(g_c), C.byref(m_c[s]), stream.ptr),
+                      "zero")
         if eng is not None:
             due = sort_every > 0 and state["phase"] > 0 and state["phase"] % sort_every == 0
             if due:
@@ -253,7 +257,7 @@ def run_stress(warmup: int) -> dict:
     species at 5x5x5 per cell plus four uniform beams (electrons and ions,
     x drift +-0.1, thermal speed 0.02) at 1x5x5 = 25 per cell, 314,572,800
     particles in FP32; the engine rebuilds its tile segments every 5 phases;
-    50 cycles (zero the moments, engine phase) are captured in one HIP graph
+    50 cycles (engine phases that write the moments) are captured in one HIP graph
     and timed as one launch."""
     import ctypes as C
 
@@ -266,7 +270,7 @@ def run_stress(warmup: int) -> dict:
     field = S.gem_fields(grd, gem)
     species = S.harris_species(PPC) + S.beam_species()
     ns = len(species)
-    eng = S.Engine(grd, ns, sort_every=STRESS_SORT_EVERY)
+    eng = S.Engine(grd, ns, sort_every=STRESS_SORT_EVERY, set_moments=True)
     stream = S.Stream()
     n_total = 0
     for s, sp in enumerate(species):
@@ -282,7 +286,6 @@ def run_stress(warmup: int) -> dict:
     param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
 
     def cycle():
-        S.setZeroSpeciesDensities(moms, grd, ns, stream)
         eng.phase(field, moms, param, stream)
 
     S.set_async(True)
@@ -406,8 +409,9 @@ def run_batch(steps: int, warmup: int) -> dict:
     """The batching form: 16 independent Harris-sheet instances (batch_specs)
     in FP32, advanced by one batch (one launch per kernel and phase); tile
     shape and sort schedule from the autotuner, timed on the same instances
-    built afresh; then `steps` cycles (zero every instance's moments, batch
-    phase) captured in one HIP graph and timed as one launch."""
+    built afresh; then `steps` cycles captured in one HIP graph and timed as
+    one launch.  A cycle is one batch phase that writes every instance's
+    moments (the zeroing before the deposit folded into the reduce)."""
     import sputnipic_amd as S
     from sputnipic_amd.tune import tune_batch
 
@@ -418,18 +422,17 @@ def run_batch(steps: int, warmup: int) -> dict:
     stream = S.Stream()
     t0 = time.perf_counter()
     tuned = tune_batch(lambda tile: [GemInstance(sp, tile, stream) for sp in specs],
-                       phases=6, warmup=1, stream=stream)
+                       phases=6, warmup=1, stream=stream, set_moments=True)
     t_tune = time.perf_counter() - t0
     log(f"batch: tuned in {t_tune:.1f} s: " + ", ".join(
         f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}: {t.ms_per_phase:.3f} ms"
         for t in tuned.trials))
     insts = [GemInstance(sp, tuned.tile, stream) for sp in specs]
     assert sum(x.n for x in insts) == n_total
-    batch = S.Batch([x.member() for x in insts], sort_every=tuned.sort_every)
+    batch = S.Batch([x.member() for x in insts], sort_every=tuned.sort_every,
+                    set_moments=True)
 
     def cycle():
-        for x in insts:
-            x.zero(stream)
         batch.phase(stream)
 
     S.set_async(True)

@@ -186,6 +186,14 @@ typedef struct spic_engine_opts {
   int slack_pct;              /* free slots per segment after a sort, in
                                  percent of the mean particles per tile
                                  (at least 16 slots); 0 = default (12) */
+  int set_moments;            /* 1: each phase WRITES the ten moments of
+                                 every loaded species (the deposit of this
+                                 phase alone, zero where no particle
+                                 deposits), which spares the caller the
+                                 zeroing that precedes interpP2G
+                                 (src/sputniPIC_CPU.cpp:247; rhoc is not
+                                 touched); 0 (default): ADD them, as
+                                 interpP2G does */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {
@@ -250,13 +258,18 @@ typedef struct spic_batch_member {
   const spic_push *push;
 } spic_batch_member;
 
+/* Batch flags: SPIC_BATCH_SET_MOMENTS makes each phase WRITE the bound
+ * moment sets (as spic_engine_opts.set_moments does for one engine). */
+#define SPIC_BATCH_SET_MOMENTS 1
+
 /* Bind n (1..SPIC_MAX_INSTANCES) distinct engines.  sort_every > 0 sorts
  * every species of every member before every sort_every-th phase; the
- * members' own sort_every does not apply to batch phases. */
+ * members' own sort_every and set_moments do not apply to batch phases. */
 int spic_batch_create(spic_batch **batch, const spic_batch_member *members,
-                      int n, int sort_every);
+                      int n, int sort_every, int flags);
 /* One particle phase of every member: mover_PC then interpP2G of each
- * species, ADDING the moments into the bound moment sets. */
+ * species, ADDING the moments into the bound moment sets (or writing
+ * them, with SPIC_BATCH_SET_MOMENTS). */
 int spic_batch_phase(spic_batch *batch, void *stream);
 int spic_batch_destroy(spic_batch *batch);
 

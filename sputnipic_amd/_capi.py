@@ -83,7 +83,7 @@ class spic_gem_species(C.Structure):
 class spic_engine_opts(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
                 ("sort_every", C.c_int), ("lds_partials", C.c_int),
-                ("slack_pct", C.c_int)]
+                ("slack_pct", C.c_int), ("set_moments", C.c_int)]
 
 
 class spic_engine_stats(C.Structure):
@@ -126,7 +126,7 @@ SIGNATURES = {
     "spic_engine_sort": (C.c_int, [_P, _P]),
     "spic_engine_get_stats": (C.c_int, [_P, C.POINTER(spic_engine_stats)]),
     "spic_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_batch_member), C.c_int,
-                                    C.c_int]),
+                                    C.c_int, C.c_int]),
     "spic_batch_phase": (C.c_int, [_P, _P]),
     "spic_batch_destroy": (C.c_int, [_P]),
     "spic_gem_fields": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_gem_params),

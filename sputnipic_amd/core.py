@@ -506,13 +506,13 @@ class Engine:
     interpP2G in one launch over HBM-resident particles binned by tile."""
 
     def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
-                 lds_partials: int = 0, slack_pct: int = 0):
+                 lds_partials: int = 0, slack_pct: int = 0, set_moments: bool = False):
         from ._capi import spic_engine_opts
         self.grd = grd
         self.ns = ns
         p = C.c_void_p()
         opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials,
-                                slack_pct)
+                                slack_pct, int(bool(set_moments)))
         g = grd.c_struct()
         check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
               "spic_engine_create")
@@ -557,9 +557,10 @@ class Batch:
 
     members: (engine, field, moment sets, parameters) per instance, the
     engines' species already loaded.  The descriptors are bound here; the
-    arrays behind them must outlive the batch."""
+    arrays behind them must outlive the batch.  set_moments: each phase
+    writes the moments instead of adding to them (no zeroing needed)."""
 
-    def __init__(self, members, sort_every: int = 0):
+    def __init__(self, members, sort_every: int = 0, set_moments: bool = False):
         from ._capi import spic_batch_member
         n = len(members)
         self._keep = []
@@ -574,8 +575,10 @@ class Batch:
             arr[i].moments = C.cast(moms, C.POINTER(spic_moments))
             arr[i].push = C.pointer(p)
         self.engines = [m[0] for m in members]
+        self.set_moments = bool(set_moments)
         ptr = C.c_void_p()
-        check(lib().spic_batch_create(C.byref(ptr), arr, n, sort_every), "spic_batch_create")
+        check(lib().spic_batch_create(C.byref(ptr), arr, n, sort_every, int(bool(set_moments))),
+              "spic_batch_create")
         self.ptr = ptr.value
 
     def phase(self, stream: Optional["Stream"] = None) -> None:

@@ -133,6 +133,9 @@ struct CountArrays {
   unsigned int *p[kMaxSpecies];
   int n;  // entries per array
   int ns;
+  // nn > 0: also zero the ten moment arrays of nn nodes of each species
+  KMom mom[kMaxSpecies];
+  int64_t nn;
 };
 
 #define SPIC_DECLARE_BINNED_LAUNCHERS(NS)                                  \
@@ -168,20 +171,21 @@ struct BinLayout {
 // Sums the tile partials of every species into its node moments (+=).
 hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int tz,
                                 int nn, hipStream_t s);
-// Zeroes ns count arrays of n entries each.
+// Zeroes ns count arrays of n entries each (and the moments, c.nn > 0).
 hipError_t zero_counts(const CountArrays &c, hipStream_t s);
 
 // The reduce and the count reset of a batch, from the device-resident
 // argument blocks tab[i] of its instances: per instance, the node blocks
 // nodes.start[i].. of 256 nodes each (reduce) and the counter blocks
 // counts.start[i].. of 256 entries each (reset of every species' cnt_out,
-// 3 * (ntiles + 1) entries); max_ns species per launch row.
+// 3 * (ntiles + 1) entries, then with `moments` the ten moment arrays of
+// every species); max_ns species per launch row.
 template <typename T>
 hipError_t launch_batch_reduce(const BinnedArgs<T> *tab, const BatchMap &nodes,
                                int max_ns, int tx, int ty, int tz, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_zero_counts(const BinnedArgs<T> *tab, const BatchMap &counts,
-                                    int max_ns, hipStream_t s);
+                                    int max_ns, int moments, hipStream_t s);
 
 // Sort of n particles of one species into binned storage (spic_binsort.hip).
 // The source is dense (src_seg == nullptr: particles 0..n-1, ids assigned

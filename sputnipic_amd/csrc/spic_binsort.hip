@@ -160,12 +160,21 @@ __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
   }
 }
 
+// Per species (blockIdx.y): the n counters, then (c.nn > 0) the ten
+// moment arrays of nn nodes each, zeroed.
 __global__ void k_zero_counts(CountArrays c) {
   const int s = blockIdx.y;
   if (s >= c.ns) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n;
-       i += gridDim.x * blockDim.x)
-    c.p[s][i] = 0u;
+  const int64_t total = c.n + 10 * c.nn;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < c.n) {
+      gmem(c.p[s])[i] = 0u;
+    } else {
+      const int64_t j = i - c.n;
+      gmem(c.mom[s].m[j / c.nn])[j % c.nn] = 0.0;
+    }
+  }
 }
 
 // Per node and species: the sum of the partials of every tile whose staged
@@ -235,16 +244,25 @@ __global__ __launch_bounds__(256) void k_binned_reduce_batch(
                           a.partials, a.sp[s].seg, a.sp[s].cnt_in, a.sp[s].m.m);
 }
 
-// The count reset of a batch: blockIdx.y is the species.
+// The count reset of a batch (blockIdx.y is the species), with the moment
+// arrays after the counters when `moments` is set: element j of instance i
+// is counter j, or node (j - counters) % nn of moment (j - counters) / nn.
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_counts_batch(
-    const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
+    const BinnedArgs<T> *__restrict__ tab, BatchMap map, int moments) {
   int i = 0;
   for (int k = 1; k < map.n; ++k) i += (int)blockIdx.x >= map.start[k] ? 1 : 0;
   const BinnedArgs<T> &a = tab[i];
   const int s = blockIdx.y;
-  const int j = (blockIdx.x - map.start[i]) * 256 + threadIdx.x;
-  if (s < a.ns && j < 3 * (a.ntiles + 1)) gmem(a.sp[s].cnt_out)[j] = 0u;
+  if (s >= a.ns) return;
+  const int64_t j = (int64_t)(blockIdx.x - map.start[i]) * 256 + threadIdx.x;
+  const int64_t nc = 3 * (int64_t)(a.ntiles + 1);
+  const int64_t nn = (int64_t)a.g.nxn * a.g.nyn * a.g.nzn;
+  if (j < nc) {
+    gmem(a.sp[s].cnt_out)[j] = 0u;
+  } else if (moments && j < nc + 10 * nn) {
+    gmem(a.sp[s].m.m[(j - nc) / nn])[(j - nc) % nn] = 0.0;
+  }
 }
 
 }  // namespace
@@ -280,10 +298,10 @@ hipError_t launch_batch_reduce(const BinnedArgs<T> *tab, const BatchMap &nodes,
 
 template <typename T>
 hipError_t launch_batch_zero_counts(const BinnedArgs<T> *tab, const BatchMap &counts,
-                                    int max_ns, hipStream_t s) {
+                                    int max_ns, int moments, hipStream_t s) {
   if (counts.n <= 0 || counts.start[counts.n] <= 0 || max_ns <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_zero_counts_batch<T>), dim3(counts.start[counts.n], max_ns),
-                     dim3(256), 0, s, tab, counts);
+                     dim3(256), 0, s, tab, counts, moments);
   return hipGetLastError();
 }
 
@@ -293,13 +311,15 @@ template hipError_t launch_batch_reduce<double>(const BinnedArgs<double> *,
 template hipError_t launch_batch_reduce<float>(const BinnedArgs<float> *, const BatchMap &,
                                                int, int, int, int, hipStream_t);
 template hipError_t launch_batch_zero_counts<double>(const BinnedArgs<double> *,
-                                                     const BatchMap &, int, hipStream_t);
+                                                     const BatchMap &, int, int,
+                                                     hipStream_t);
 template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
-                                                    const BatchMap &, int, hipStream_t);
+                                                    const BatchMap &, int, int, hipStream_t);
 
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n), c.ns), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n + 10 * c.nn), c.ns), dim3(256), 0, s,
+                     c);
   return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ struct spic_engine {
   int slack_pct = 12;
   int sort_every = 0;
   int lds_fp64 = 1;
+  int set_moments = 0;             // phases write the moments (else add)
   int64_t phases = 0, sorts = 0;
   uint64_t gen = 0;                // bumped by every load (storage may move)
   size_t bytes = 0;
@@ -198,9 +199,13 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     r.seg[j] = a.sp[j].seg;
     r.mom[j] = a.sp[j].m;
     zc.p[j] = a.sp[j].cnt_out;
+    zc.mom[j] = a.sp[j].m;
   }
   r.ns = zc.ns = k;
   zc.n = 3 * (e->L.ntiles + 1);
+  // writing the moments: zero them in the same launch; the deposits (LDS
+  // partials through the reduce, global atomics directly) then add
+  zc.nn = e->set_moments ? (int64_t)e->grd.nxn * e->grd.nyn * e->grd.nzn : 0;
   hipError_t err = spic::zero_counts(zc, s);
   if (err != hipSuccess) return err;
   err = spic::strict_mode() ? spic::spic_strict::launch_binned_phase<T>(a, s)
@@ -229,7 +234,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (rc) return rc;
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
-  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12;
+  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -241,6 +246,9 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->slack_pct < 0 || opts->slack_pct > 1000)
       return fail(SPIC_EINVAL, "slack_pct must be 0..1000");
     if (opts->slack_pct) slack_pct = opts->slack_pct;
+    if (opts->set_moments != 0 && opts->set_moments != 1)
+      return fail(SPIC_EINVAL, "set_moments must be 0 or 1");
+    set_mom = opts->set_moments;
   }
   // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
   if (grd->nxn > 4096 || grd->nyn > 1024 || grd->nzn > 1024)
@@ -258,6 +266,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->sort_every = sort_every;
   e->lds_fp64 = lds_fp64;
   e->slack_pct = slack_pct;
+  e->set_moments = set_mom;
   auto &L = e->L;
   L.ncx = grd->nxc - 2; L.ncy = grd->nyc - 2; L.ncz = grd->nzc - 2;
   L.tx = tx; L.ty = ty; L.tz = tz;
@@ -483,6 +492,7 @@ struct spic_batch {
   int n = 0;
   int prec = SPIC_F64;
   int sort_every = 0;
+  int flags = 0;
   int64_t phases = 0;
   spic_engine *e[spic::kMaxInstances] = {};
   spic_field field[spic::kMaxInstances];
@@ -533,8 +543,9 @@ hipError_t build_tables(spic_batch *b, unsigned int *status) {
     b->tiles.start[i + 1] = b->tiles.start[i] + (any ? e->L.ntiles : 0);
     b->ovf.start[i + 1] = b->ovf.start[i] + (any ? kBatchOverflowBlocks : 0);
     b->nodes.start[i + 1] = b->nodes.start[i] + (any ? blocks(nn, 256) : 0);
-    b->counts.start[i + 1] =
-        b->counts.start[i] + (any ? blocks(3 * (int64_t)(e->L.ntiles + 1), 256) : 0);
+    const int64_t zeroed = 3 * (int64_t)(e->L.ntiles + 1) +
+                           ((b->flags & SPIC_BATCH_SET_MOMENTS) ? 10 * nn : 0);
+    b->counts.start[i + 1] = b->counts.start[i] + (any ? blocks(zeroed, 256) : 0);
     b->gen[i] = e->gen;
   }
   b->status = status;
@@ -546,7 +557,8 @@ hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
   const spic::BinnedArgs<T> *tab =
       (const spic::BinnedArgs<T> *)((char *)b->tab + (size_t)state * b->n * b->arg_bytes);
   const spic::BinLayout &L = b->e[0]->L;
-  hipError_t err = spic::launch_batch_zero_counts<T>(tab, b->counts, b->max_ns, s);
+  const int set = (b->flags & SPIC_BATCH_SET_MOMENTS) ? 1 : 0;
+  hipError_t err = spic::launch_batch_zero_counts<T>(tab, b->counts, b->max_ns, set, s);
   if (err != hipSuccess) return err;
   err = spic::strict_mode()
             ? spic::spic_strict::launch_binned_phase_batch<T>(
@@ -562,13 +574,14 @@ hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
 extern "C" {
 
 int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
-                      int sort_every) {
+                      int sort_every, int flags) {
   if (!out) return fail(SPIC_EINVAL, "batch pointer is NULL");
   *out = nullptr;
   if (!members) return fail(SPIC_EINVAL, "batch members are NULL");
   if (n < 1 || n > spic::kMaxInstances)
     return fail(SPIC_EINVAL, "batch instance count must be 1..%d", spic::kMaxInstances);
   if (sort_every < 0) return fail(SPIC_EINVAL, "sort_every must be >= 0");
+  if (flags & ~SPIC_BATCH_SET_MOMENTS) return fail(SPIC_EINVAL, "unknown batch flags");
   int rc;
   for (int i = 0; i < n; ++i) {
     const spic_batch_member &m = members[i];
@@ -593,6 +606,7 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
   b->n = n;
   b->prec = members[0].engine->grd.prec;
   b->sort_every = sort_every;
+  b->flags = flags;
   for (int i = 0; i < n; ++i) {
     b->e[i] = members[i].engine;
     b->field[i] = *members[i].field;

@@ -7,8 +7,9 @@ Each tile shape needs its own engines, so the caller supplies a builder:
 ``make_instances(tile)`` returns fresh instances (loaded from the same
 initial condition every time) with ``member()`` (the Batch member tuple),
 ``zero(stream)`` (zero the instance's moments) and ``close()``.  Every
-candidate runs ``warmup`` untimed and ``phases`` timed cycles (zero, batch
-phase) on one stream, timed with HIP events.  The tuner advances only its
+candidate runs ``warmup`` untimed and ``phases`` timed cycles (zero unless
+the batch writes the moments, batch phase) on one stream, timed with HIP
+events.  The tuner advances only its
 own instances; the caller builds the production instances afterwards with
 the winning settings.
 """
@@ -49,11 +50,13 @@ class TuneResult:
 
 
 def time_batch(insts, batch: Batch, stream: Stream, phases: int, warmup: int) -> float:
-    """Mean milliseconds per cycle (zero the moments, batch phase), with the
-    calls in async mode so that no host wait falls inside the timed span."""
+    """Mean milliseconds per cycle (zero the moments unless the batch writes
+    them, batch phase), with the calls in async mode so that no host wait
+    falls inside the timed span."""
     def cycle():
-        for x in insts:
-            x.zero(stream)
+        if not batch.set_moments:
+            for x in insts:
+                x.zero(stream)
         batch.phase(stream)
 
     was_async = get_async()
@@ -76,7 +79,8 @@ def time_batch(insts, batch: Batch, stream: Stream, phases: int, warmup: int) ->
 def tune_batch(make_instances: Callable[[Tuple[int, int, int]], Sequence],
                tiles: Sequence[Tuple[int, int, int]] = TILES,
                sort_choices: Sequence[int] = SORT_CHOICES, phases: int = 10,
-               warmup: int = 2, stream: Optional[Stream] = None) -> TuneResult:
+               warmup: int = 2, stream: Optional[Stream] = None,
+               set_moments: bool = False) -> TuneResult:
     if not tiles or not sort_choices:
         raise ValueError("tune_batch needs at least one tile shape and one sort choice")
     if phases < 1 or warmup < 0:
@@ -87,7 +91,8 @@ def tune_batch(make_instances: Callable[[Tuple[int, int, int]], Sequence],
         insts = make_instances(tuple(tile))
         try:
             for se in sort_choices:
-                batch = Batch([x.member() for x in insts], sort_every=se)
+                batch = Batch([x.member() for x in insts], sort_every=se,
+                              set_moments=set_moments)
                 try:
                     ms = time_batch(insts, batch, stream, phases, warmup)
                 finally:

@@ -16,7 +16,7 @@ import pytest
 
 import oracle as O
 import sputnipic_amd as S
-from helpers import PRECS, assert_bitwise, assert_close
+from helpers import PRECS, assert_bitwise, assert_close, load_case, oracle_from_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -105,6 +105,61 @@ def test_batch_matches_oracle_per_instance(prec, tile, sort_every):
         assert stats["phases"] == phases
         if sort_every:
             assert stats["sorts"] == x.ns * (1 + (phases - 1) // sort_every)
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_set_moments_overwrites(prec):
+    """With set_moments the phase writes the moments: started from garbage
+    and never zeroed, every phase's moments are that phase's deposit."""
+    states = [_oracle(spec, prec) for spec in INSTANCES]
+    insts = [Instance(st, prec) for st in states]
+    rng = np.random.default_rng(5)
+    for x in insts:
+        for m in x.ids:
+            m.upload({n: rng.normal(0, 1e3, x.grd.nn) for n in O.MOMENT_NAMES})
+    batch = S.Batch([x.member() for x in insts], sort_every=2, set_moments=True)
+    for _ in range(3):
+        batch.phase()
+    for st in states:
+        for _ in range(3):
+            st.cycle()
+    _check([x.results() for x in insts], states, prec, "set_moments")
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_set_moments_keeps_global_deposits(prec):
+    """An instance whose particles spill into the overflow region (gem_hot,
+    no fields, species 0's last cell layer of tile 0 drifting +0.9 cell per
+    step into tile 1) beside a plain one: the overflow pass deposits with
+    global atomics, which the written moments must include."""
+    data = load_case("gem_hot", prec)
+    hot = oracle_from_fixture("gem_hot", prec, data)
+    for n in O.FIELD_NAMES:
+        hot.fields[n][:] = 0.0
+    c = hot.cfg
+    dx = c.Lx / c.nxc
+    for s_, p in enumerate(hot.parts):
+        for k in ("u", "v", "w"):
+            p[k][:] = 0.0
+        if s_ == 0:
+            last = (p["x"] >= 3 * dx) & (p["x"] < 4 * dx)
+            p["u"][last] = 0.9 * dx / c.dt
+    states = [hot, _oracle(INSTANCES[1], prec)]
+    insts = [Instance(st, prec) for st in states]
+    batch = S.Batch([x.member() for x in insts], set_moments=True)
+    for _ in range(3):
+        batch.phase()
+    for st in states:
+        for _ in range(3):
+            st.cycle()
+    assert insts[0].eng.stats()["global_deposits"] > 0
+    _check([x.results() for x in insts], states, prec, "set_moments overflow")
     batch.close()
     for x in insts:
         x.eng.close()

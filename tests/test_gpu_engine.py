@@ -43,7 +43,7 @@ def oracle_small(prec):
 
 
 def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0,
-               slack_pct=0):
+               slack_pct=0, set_moments=False):
     c = st.cfg
     grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
     dt = np.float64 if prec == "f64" else np.float32
@@ -52,11 +52,18 @@ def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0
     ids = [S.InterpDensSpecies(grd) for _ in range(c.ns)]
     param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
                          PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
-    eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every, slack_pct=slack_pct)
+    eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every, slack_pct=slack_pct,
+                   set_moments=set_moments)
     for s, p in enumerate(parts):
         eng.load(s, p)
+    if set_moments:
+        # the phase writes the moments: start from garbage, never zero
+        rng = np.random.default_rng(11)
+        for m in ids:
+            m.upload({n: rng.normal(0, 1e3, grd.nn) for n in O.MOMENT_NAMES})
     for _ in range(phases):
-        S.setZeroSpeciesDensities(ids, grd, c.ns)
+        if not set_moments:
+            S.setZeroSpeciesDensities(ids, grd, c.ns)
         eng.phase(field, ids, param)
     for s, p in enumerate(parts):
         eng.store(s, p)
@@ -79,6 +86,20 @@ def test_engine_strict_matches_oracle_bitwise(prec):
     got_p, got_m, stats = run_engine(st, prec, phases=3)
     oracle_phases(st, 3)
     assert stats["phases"] == 3
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_set_moments_matches_oracle(prec):
+    st = oracle_small(prec)
+    S.set_strict_fp(True)
+    got_p, got_m, _ = run_engine(st, prec, phases=2, sort_every=1, set_moments=True)
+    oracle_phases(st, 2)
     for s in range(st.cfg.ns):
         for k in O.PARTICLE_NAMES:
             assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
@@ -141,13 +162,16 @@ def test_engine_fast_particles_cross_tiles_every_phase():
             assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
 
 
+@pytest.mark.parametrize("set_moments", [False, True])
 @pytest.mark.parametrize("prec", PRECS)
-def test_engine_overflow_region(prec):
+def test_engine_overflow_region(prec, set_moments):
     """A full segment spills into the overflow region, which the next phase
     pushes with the global-memory kernel.  gem_hot grid (8x4x4 cells, two
     4x4x4 tiles along x) with no fields: species 0's particles in the last
     cell layer of tile 0 drift +0.9 cell per step into tile 1, all others
-    rest, so tile 1 gains 32 particles against 16 free slots."""
+    rest, so tile 1 gains 32 particles against 16 free slots.  With
+    set_moments the phase writes the moments, global-atomic deposits
+    included."""
     data = load_case("gem_hot", prec)
     st = oracle_from_fixture("gem_hot", prec, data)
     for n in O.FIELD_NAMES:
@@ -162,7 +186,7 @@ def test_engine_overflow_region(prec):
             p["u"][last] = 0.9 * dx / c.dt
             assert int(np.sum(last)) == 32
     S.set_strict_fp(True)
-    got_p, got_m, stats = run_engine(st, prec, phases=3)
+    got_p, got_m, stats = run_engine(st, prec, phases=3, set_moments=set_moments)
     oracle_phases(st, 3)
     assert stats["overflow"] > 0, stats
     assert stats["global_deposits"] > 0, stats
