@@ -160,6 +160,65 @@ int spic_sum_species(const spic_grid *grd, const spic_moments *net,
                      const spic_moments *ids, int ns, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Field-solve side of the cycle, on the device, FP64 (grid prec F64):     */
+/* the consumers of the deposited moments before and inside the implicit   */
+/* field solve.  Bit-identical to the reference's FP64 build.  Arrays are  */
+/* FP64 device arrays: nodes [nxn][nyn][nzn], cells [nxc][nyc][nzc]       */
+/* (guard-inclusive), k fastest.                                           */
+/* ---------------------------------------------------------------------- */
+
+/* The members of struct parameters these operators read
+ * (include/Parameters.h): dt, c, th (theta), fourpi, the field periodicity
+ * flags and the species' qom. */
+typedef struct spic_solve_params {
+  double dt, c, th, fourpi;
+  int PERIODICX, PERIODICY, PERIODICZ;
+  double qom[8];
+} spic_solve_params;
+
+/* struct interpDens_aux (include/InterpDens_aux.h:6-13): rho hat on the
+ * cells, J hat on the nodes. */
+typedef struct spic_hat {
+  double *rhoh;
+  double *Jxh, *Jyh, *Jzh;
+} spic_hat;
+
+/* Workspace bytes that spic_hat_densities and spic_maxwell_image need for
+ * ns species on grd. */
+size_t spic_field_workspace_bytes(const spic_grid *grd, int ns);
+
+/* Ghost update of one scalar array, in place: kind 0 applyBCscalarDensN
+ * (nodes, src/BC.cpp:11-123), 1 applyBCscalarFieldNzero (nodes, zeroed
+ * non-periodic faces, src/BC.cpp:242-352), 2 applyBCscalarDensC (cells,
+ * src/BC.cpp:623-735), 3 applyBCscalarFieldC (cells, src/BC.cpp:737-850). */
+int spic_bc_scalar(const spic_grid *grd, double *a, int kind,
+                   const spic_solve_params *p, void *stream);
+
+/* The driver's step before the solver (src/sputniPIC_CPU.cpp:276-277):
+ * applyBCscalarDensN on the net rhon (in place), then rhoc =
+ * interpN2Cinterp(rhon) (src/Grid.cpp:164-185) on the interior cells. */
+int spic_rhoc_from_rhon(const spic_grid *grd, double *rhon, double *rhoc,
+                        const spic_solve_params *p, void *stream);
+
+/* calculateHatDensities (src/MaxwellSolver.cpp:465-604): from the ns
+ * (1..8) species' currents and pressure tensors, the net rhoc and B,
+ * writes J hat (nodes) and rho hat (cells), ghosts included. */
+int spic_hat_densities(const spic_grid *grd, const spic_field *field,
+                       const spic_moments *ids, int ns, const double *rhoc,
+                       const spic_solve_params *p, const spic_hat *out,
+                       void *work, size_t work_bytes, void *stream);
+
+/* MaxwellImage (src/MaxwellSolver.cpp:95-257): im = A vec for the GMRES
+ * field solve; vec and im hold the interior nodes' (x, y, z) components
+ * interleaved, i slowest (solver2phys / phys2solver, src/Basic.cpp:
+ * 365-392), 3 (nxn-2)(nyn-2)(nzn-2) values; ns (0..8) species' rhon. */
+int spic_maxwell_image(const spic_grid *grd, const spic_field *field,
+                       const spic_moments *ids, int ns,
+                       const spic_solve_params *p, const double *vec,
+                       double *im, void *work, size_t work_bytes,
+                       void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Tile-resident engine: the whole particle phase of a cycle -- mover_PC   */
 /* then interpP2G for every species, the loop of                          */
 /* src/sputniPIC_CPU.cpp:232-256 -- in one launch over HBM-resident        */

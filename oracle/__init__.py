@@ -20,6 +20,12 @@ Two libraries are wrapped:
   gem_particles        src/IC.cpp:117-177
   apply_bc_ids         src/BC.cpp:362-619 (applyBCids)
   sum_species          src/InterpDensNet.cpp:120-144 (sumOverSpecies)
+  bc_scalar            src/BC.cpp:11-123, 242-352, 623-735, 737-850
+                       (applyBCscalarDensN / FieldNzero / DensC / FieldC)
+  rhoc_from_rhon       src/sputniPIC_CPU.cpp:276-277 (applyBCscalarDensN,
+                       interpN2Cinterp src/Grid.cpp:164-185)
+  hat_densities        src/MaxwellSolver.cpp:465-604 (calculateHatDensities)
+  maxwell_image        src/MaxwellSolver.cpp:95-257 (MaxwellImage)
   ===================  ===================================================
 
   How each is pinned (tests/test_oracle.py): every function is compared bit
@@ -29,11 +35,14 @@ Two libraries are wrapped:
   (free streaming, uniform-E kick, uniform-B rotation, exact trilinear
   reproduction of linear fields); interp_p2g by charge / current / dipole
   conservation and a long-double brute force; particle_sort by permutation
-  and key-order invariants.
+  and key-order invariants; hat_densities and maxwell_image by the
+  reference's own MaxwellSolver.cpp in both precisions, for every
+  periodicity.
 
 * ``_ref/libref_{f64,f32}.so`` -- the reference's own ``src/Mover.cpp``,
   ``Grid.cpp``, ``Particles.cpp``, ``IC.cpp``, ``InterpDensSpecies.cpp``,
-  ``InterpDensNet.cpp``, ``EMfield.cpp`` and ``BC.cpp`` compiled unmodified
+  ``InterpDensNet.cpp``, ``EMfield.cpp``, ``BC.cpp``, ``MaxwellSolver.cpp``,
+  ``Basic.cpp`` and ``Solvers.cpp`` compiled unmodified
   from the reference checkout (``oracle/Makefile``) with the test harness in
   ``ref_harness.cpp``.  Present only where the reference checkout exists
   (this build container); never on the GPU machines.
@@ -154,6 +163,18 @@ def lib():
             f.argtypes = [C.POINTER(OGrid), C.POINTER(OMoments), C.c_int, C.c_int, C.c_int]
             f = getattr(L, f"oracle_sum_species_{sfx}")
             f.argtypes = [C.POINTER(OGrid), C.POINTER(OMoments), C.POINTER(OMoments), C.c_int]
+            f = getattr(L, f"oracle_bc_scalar_{sfx}")
+            f.argtypes = [C.POINTER(OGrid), _P, C.c_int, C.c_int, C.c_int, C.c_int]
+            f = getattr(L, f"oracle_rhoc_from_rhon_{sfx}")
+            f.argtypes = [C.POINTER(OGrid), _P, _P, C.c_int, C.c_int, C.c_int]
+            f = getattr(L, f"oracle_hat_densities_{sfx}")
+            f.argtypes = [C.POINTER(OGrid), C.POINTER(OMoments), C.c_int, _P, _P, _P, _P, _P,
+                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                          _P, _P, _P, _P]
+            f = getattr(L, f"oracle_maxwell_image_{sfx}")
+            f.argtypes = [C.POINTER(OGrid), _P, _P, _P, C.POINTER(OMoments), C.c_int, _P,
+                          C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                          C.c_int, _P, _P]
             f = getattr(L, f"oracle_zero_moments_{sfx}")
             f.argtypes = [C.POINTER(OGrid), C.POINTER(OMoments)]
             f = getattr(L, f"oracle_time_cycles_{sfx}")
@@ -203,6 +224,7 @@ class GemConfig:
     delta: float = 0.5
     fourpi: float = 12.5663706144
     pertX: float = 0.4
+    th: float = 1.0
     PERIODICX: bool = True
     PERIODICY: bool = False
     PERIODICZ: bool = True
@@ -353,6 +375,61 @@ class State:
                                                            ids, ns_)
         return net
 
+    # -- field-solve side ----------------------------------------------------
+    def bc_scalar(self, a: np.ndarray, kind: int) -> np.ndarray:
+        """Ghost update of a copy of `a` (kind 0 DensN, 1 FieldNzero, 2
+        DensC, 3 FieldC)."""
+        a = np.ascontiguousarray(a, self.dt).copy()
+        c = self.cfg
+        getattr(lib(), f"oracle_bc_scalar_{self.prec}")(
+            C.byref(self.grid), _ptr(a), kind, int(c.PERIODICX), int(c.PERIODICY),
+            int(c.PERIODICZ))
+        return a
+
+    def rhoc_from_rhon(self, rhon: np.ndarray):
+        """(rhon after applyBCscalarDensN, rhoc = interpN2Cinterp(rhon)); the
+        ghost cells of rhoc are zero."""
+        rhon = np.ascontiguousarray(rhon, self.dt).copy()
+        rhoc = np.zeros(self.ncell, self.dt)
+        c = self.cfg
+        getattr(lib(), f"oracle_rhoc_from_rhon_{self.prec}")(
+            C.byref(self.grid), _ptr(rhoc), _ptr(rhon), int(c.PERIODICX), int(c.PERIODICY),
+            int(c.PERIODICZ))
+        return rhon, rhoc
+
+    def hat_densities(self, rhoc: np.ndarray) -> Dict[str, np.ndarray]:
+        """calculateHatDensities from this state's species moments and B and
+        the given net rhoc: rhoh (cells), Jxh Jyh Jzh (nodes)."""
+        c = self.cfg
+        ns_ = c.ns
+        ids = (OMoments * ns_)(*[self.mom_struct(i) for i in range(ns_)])
+        qom = np.array(c.qom[:ns_], np.float64)
+        rhoc = np.ascontiguousarray(rhoc, self.dt)
+        out = {"rhoh": np.zeros(self.ncell, self.dt)}
+        for n in ("Jxh", "Jyh", "Jzh"):
+            out[n] = np.zeros(self.nn, self.dt)
+        getattr(lib(), f"oracle_hat_densities_{self.prec}")(
+            C.byref(self.grid), ids, ns_, _ptr(qom), _ptr(self.fields["Bxn"]),
+            _ptr(self.fields["Byn"]), _ptr(self.fields["Bzn"]), _ptr(rhoc), c.dt, c.c, c.th,
+            int(c.PERIODICX), int(c.PERIODICY), int(c.PERIODICZ), _ptr(out["rhoh"]),
+            _ptr(out["Jxh"]), _ptr(out["Jyh"]), _ptr(out["Jzh"]))
+        return out
+
+    def maxwell_image(self, vec: np.ndarray) -> np.ndarray:
+        """MaxwellImage of a Krylov vector (3 (nxn-2)(nyn-2)(nzn-2) values)
+        with this state's species rhon and B."""
+        c = self.cfg
+        ns_ = c.ns
+        ids = (OMoments * ns_)(*[self.mom_struct(i) for i in range(ns_)])
+        qom = np.array(c.qom[:ns_], np.float64)
+        vec = np.ascontiguousarray(vec, self.dt)
+        im = np.zeros_like(vec)
+        getattr(lib(), f"oracle_maxwell_image_{self.prec}")(
+            C.byref(self.grid), _ptr(self.fields["Bxn"]), _ptr(self.fields["Byn"]),
+            _ptr(self.fields["Bzn"]), ids, ns_, _ptr(qom), c.dt, c.c, c.th, c.fourpi,
+            int(c.PERIODICX), int(c.PERIODICY), int(c.PERIODICZ), _ptr(vec), _ptr(im))
+        return im
+
     def cycle(self) -> None:
         """One cycle of the hot path as src/sputniPIC_CPU.cpp:232-256 runs
         it: mover_PC for every species, zero the moments, interpP2G for every
@@ -421,8 +498,10 @@ class RefState:
     def __init__(self, cfg: GemConfig, prec: str = "f64"):
         if cfg.ns > 6:
             raise ValueError("the reference holds at most NS_MAX = 6 species")
-        if cfg.seed_base != 0 or cfg.pertX != 0.4 or cfg.fourpi != 12.5663706144:
-            raise ValueError("the reference IC is fixed to seed 0, pertX 0.4, default fourpi")
+        if (cfg.seed_base != 0 or cfg.pertX != 0.4 or cfg.fourpi != 12.5663706144
+                or cfg.th != 1.0):
+            raise ValueError("the reference IC is fixed to seed 0, pertX 0.4, default fourpi "
+                             "and th")
         self.cfg, self.prec = cfg, prec
         L = self._load(prec)
         rc = RefCfg()
@@ -481,6 +560,8 @@ class RefState:
                 getattr(L, f).argtypes = [_P, C.c_int]
             L.ref_zero_species_moments.argtypes = [_P]
             L.ref_sum_species.argtypes = [_P, C.POINTER(C.c_double)]
+            L.ref_hat_densities.argtypes = [_P, _P]
+            L.ref_maxwell_image.argtypes = [_P, _P, _P]
             L.ref_time_cycles.argtypes = [_P, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
             L.ref_time_cycles.restype = C.c_double
@@ -559,3 +640,24 @@ class RefState:
         out = np.zeros(10 * nn, np.float64)
         self.L.ref_sum_species(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {n: out[i * nn:(i + 1) * nn].copy() for i, n in enumerate(MOMENT_NAMES)}
+
+    def hat_densities(self) -> Dict[str, np.ndarray]:
+        """The reference's sumOverSpecies, applyBCscalarDensN(net rhon),
+        interpN2Cinterp(net rhoc) and calculateHatDensities: rhon (after the
+        BC), rhoc, rhoh, Jxh, Jyh, Jzh (float64)."""
+        nn, nc = self.nn, self.ncell
+        out = np.zeros(4 * nn + 2 * nc, np.float64)
+        self.L.ref_hat_densities(self.h, _ptr(out))
+        sizes = [("rhon", nn), ("rhoc", nc), ("rhoh", nc), ("Jxh", nn), ("Jyh", nn),
+                 ("Jzh", nn)]
+        res, o = {}, 0
+        for name, n in sizes:
+            res[name] = out[o:o + n].copy()
+            o += n
+        return res
+
+    def maxwell_image(self, vec: np.ndarray) -> np.ndarray:
+        vec = np.ascontiguousarray(vec, np.float64)
+        im = np.zeros_like(vec)
+        self.L.ref_maxwell_image(self.h, _ptr(vec), _ptr(im))
+        return im

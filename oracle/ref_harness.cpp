@@ -11,7 +11,9 @@
 //   initGEM (src/IC.cpp:7), mover_PC / interpP2G / mover_interp
 //   (src/Mover.cpp), particle_sort (src/Particles.cpp:72), applyBCids
 //   (src/BC.cpp:362), setZeroSpeciesDensities (src/InterpDensNet.cpp:40),
-//   setZeroNetDensities / sumOverSpecies (src/InterpDensNet.cpp:78-144).
+//   setZeroNetDensities / sumOverSpecies (src/InterpDensNet.cpp:78-144),
+//   applyBCscalarDensN (src/BC.cpp:11), interpN2Cinterp (src/Grid.cpp:164),
+//   calculateHatDensities / MaxwellImage (src/MaxwellSolver.cpp:465, 95).
 // It fills struct parameters directly, with the defaults of
 // src/RW_IO.cpp:37-295 for keys it does not set, because readInputFile
 // lives in RW_IO.cpp, which needs HDF5 (absent here).
@@ -25,6 +27,8 @@
 #include "IC.h"
 #include "InterpDensNet.h"
 #include "InterpDensSpecies.h"
+#include "InterpDens_aux.h"
+#include "MaxwellSolver.h"
 #include "Mover.h"
 #include "Parameters.h"
 #include "Particles.h"
@@ -273,6 +277,48 @@ void ref_sum_species(void *h, double *out) {
                      idn.pyz_flat,  idn.pzz_flat};
   for (int m = 0; m < 10; ++m) to_double(a[m], out + m * nn, nn);
   interp_dens_net_deallocate(&s->grd, &idn);
+}
+
+// The field-solve side's first steps (src/sputniPIC_CPU.cpp:258-281):
+// sumOverSpecies into a zeroed net, applyBCscalarDensN on the net rhon,
+// interpN2Cinterp into the net rhoc, then calculateHatDensities.  out
+// receives rhon (nn, after the BC), rhoc (nc), rhoh (nc), Jxh Jyh Jzh (nn
+// each), in that order.
+void ref_hat_densities(void *h, double *out) {
+  ref_state *s = (ref_state *)h;
+  interpDensNet idn;
+  interp_dens_net_allocate(&s->grd, &idn);
+  setZeroNetDensities(&idn, &s->grd);
+  sumOverSpecies(&idn, s->ids, &s->grd, s->param.ns);
+  applyBCscalarDensN(idn.rhon, &s->grd, &s->param);
+  interpN2Cinterp(idn.rhoc, idn.rhon, &s->grd);
+  interpDens_aux id_aux;
+  interp_dens_aux_allocate(&s->grd, &id_aux);
+  calculateHatDensities(&id_aux, &idn, s->ids, &s->field, &s->grd, &s->param);
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  long nc = (long)s->grd.nxc * s->grd.nyc * s->grd.nzc;
+  to_double(idn.rhon_flat, out, nn);
+  to_double(&idn.rhoc[0][0][0], out + nn, nc);
+  to_double(&id_aux.rhoh[0][0][0], out + nn + nc, nc);
+  to_double(&id_aux.Jxh[0][0][0], out + nn + 2 * nc, nn);
+  to_double(&id_aux.Jyh[0][0][0], out + 2 * nn + 2 * nc, nn);
+  to_double(&id_aux.Jzh[0][0][0], out + 3 * nn + 2 * nc, nn);
+  interp_dens_aux_deallocate(&s->grd, &id_aux);
+  interp_dens_net_deallocate(&s->grd, &idn);
+}
+
+// MaxwellImage of vec (3 (nxn-2)(nyn-2)(nzn-2) values) into im, with the
+// species' rhon as they are.
+void ref_maxwell_image(void *h, const double *vec, double *im) {
+  ref_state *s = (ref_state *)h;
+  long n = 3L * (s->grd.nxn - 2) * (s->grd.nyn - 2) * (s->grd.nzn - 2);
+  FPfield *v = new FPfield[n];
+  FPfield *r = new FPfield[n];
+  from_double(vec, v, n);
+  MaxwellImage(r, v, &s->field, s->ids, &s->grd, &s->param);
+  to_double(r, im, n);
+  delete[] v;
+  delete[] r;
 }
 
 }  // extern "C"

@@ -17,6 +17,12 @@
  *                        (initGEM fields/densities/particles, interpN2Crho)
  *   oracle_apply_bc_ids  src/BC.cpp:362-619       (applyBCids)
  *   oracle_sum_species   src/InterpDensNet.cpp:120-144 (sumOverSpecies)
+ *   oracle_bc_scalar     src/BC.cpp:11-123, 242-352, 623-735, 737-850
+ *                        (applyBCscalarDensN / FieldNzero / DensC / FieldC)
+ *   oracle_rhoc_from_rhon  src/sputniPIC_CPU.cpp:276-277 (applyBCscalarDensN
+ *                        on the net rhon, interpN2Cinterp src/Grid.cpp:164-185)
+ *   oracle_hat_densities src/MaxwellSolver.cpp:465-604 (calculateHatDensities)
+ *   oracle_maxwell_image src/MaxwellSolver.cpp:95-257  (MaxwellImage)
  *
  * The file is compiled twice.  With -DSPIC_ORACLE_FP64 the three precision
  * typedefs are double (the reference built with include/PrecisionTypes.h set
@@ -630,6 +636,392 @@ void OSFX(oracle_sum_species)(const oracle_grid *grd, oracle_moments *net,
     for (long n = 0; n < nn; n++)
       for (int m = 0; m < 10; m++) dst[m][n] += src[m][n];
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Ghost updates of one scalar array: the four routines of src/BC.cpp  */
+/*   applyBCscalarDensN      (BC.cpp:11-123)   nodes, copied faces     */
+/*   applyBCscalarFieldNzero (BC.cpp:242-352)  nodes, zeroed faces     */
+/*   applyBCscalarDensC      (BC.cpp:623-735)  cells, copied faces     */
+/*   applyBCscalarFieldC     (BC.cpp:737-850)  cells, copied faces     */
+/* share one sequence: periodic faces x, y, z; edges along x, y, z;    */
+/* the eight corners; non-periodic faces x, y, z.  A node array takes  */
+/* its ghosts from planes n-3 and 2, a cell array from n-2 and 1.  All */
+/* four run the y-edge loop to the node count (j < nyn - 1), which for */
+/* a cell array reaches the ghost layer j = nyc - 1.                   */
+/* ------------------------------------------------------------------ */
+static void bc_scalar(FPinterp *a, int n0, int n1, int n2, int nyn, int nodes,
+                      int zero_faces, int PX, int PY, int PZ) {
+  const int lo = nodes ? 3 : 2, hi = nodes ? 2 : 1;
+#define A(i, j, k) a[IDX3(i, j, k, n1, n2)]
+  if (PX)
+    for (int j = 1; j < n1 - 1; j++)
+      for (int k = 1; k < n2 - 1; k++) {
+        A(0, j, k) = A(n0 - lo, j, k);
+        A(n0 - 1, j, k) = A(hi, j, k);
+      }
+  if (PY)
+    for (int i = 1; i < n0 - 1; i++)
+      for (int k = 1; k < n2 - 1; k++) {
+        A(i, 0, k) = A(i, n1 - lo, k);
+        A(i, n1 - 1, k) = A(i, hi, k);
+      }
+  if (PZ)
+    for (int i = 1; i < n0 - 1; i++)
+      for (int j = 1; j < n1 - 1; j++) {
+        A(i, j, 0) = A(i, j, n2 - lo);
+        A(i, j, n2 - 1) = A(i, j, hi);
+      }
+  if (PY || PZ)
+    for (int i = 1; i < n0 - 1; i++) {
+      A(i, n1 - 1, n2 - 1) = A(i, hi, hi);
+      A(i, 0, 0) = A(i, n1 - lo, n2 - lo);
+      A(i, 0, n2 - 1) = A(i, n1 - lo, hi);
+      A(i, n1 - 1, 0) = A(i, hi, n2 - lo);
+    }
+  if (PX || PZ)
+    for (int j = 1; j < nyn - 1; j++) {
+      A(n0 - 1, j, n2 - 1) = A(hi, j, hi);
+      A(0, j, 0) = A(n0 - lo, j, n2 - lo);
+      A(0, j, n2 - 1) = A(n0 - lo, j, hi);
+      A(n0 - 1, j, 0) = A(hi, j, n2 - lo);
+    }
+  if (PX || PY)
+    for (int k = 1; k < n2 - 1; k++) {
+      A(n0 - 1, n1 - 1, k) = A(hi, hi, k);
+      A(0, 0, k) = A(n0 - lo, n1 - lo, k);
+      A(n0 - 1, 0, k) = A(hi, n1 - lo, k);
+      A(0, n1 - 1, k) = A(n0 - lo, hi, k);
+    }
+  if (PX || PY || PZ) {
+    A(n0 - 1, n1 - 1, n2 - 1) = A(hi, hi, hi);
+    A(0, n1 - 1, n2 - 1) = A(n0 - lo, hi, hi);
+    A(n0 - 1, 0, n2 - 1) = A(hi, n1 - lo, hi);
+    A(0, 0, n2 - 1) = A(n0 - lo, n1 - lo, hi);
+    A(n0 - 1, n1 - 1, 0) = A(hi, hi, n2 - lo);
+    A(0, n1 - 1, 0) = A(n0 - lo, hi, n2 - lo);
+    A(n0 - 1, 0, 0) = A(hi, n1 - lo, n2 - lo);
+    A(0, 0, 0) = A(n0 - lo, n1 - lo, n2 - lo);
+  }
+  if (!PX)
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n2; k++) {
+        A(0, j, k) = zero_faces ? 0.0 : A(1, j, k);
+        A(n0 - 1, j, k) = zero_faces ? 0.0 : A(n0 - 2, j, k);
+      }
+  if (!PY)
+    for (int i = 0; i < n0; i++)
+      for (int k = 0; k < n2; k++) {
+        A(i, 0, k) = zero_faces ? 0.0 : A(i, 1, k);
+        A(i, n1 - 1, k) = zero_faces ? 0.0 : A(i, n1 - 2, k);
+      }
+  if (!PZ)
+    for (int i = 0; i < n0; i++)
+      for (int j = 0; j < n1; j++) {
+        A(i, j, 0) = zero_faces ? 0.0 : A(i, j, 1);
+        A(i, j, n2 - 1) = zero_faces ? 0.0 : A(i, j, n2 - 2);
+      }
+#undef A
+}
+
+/* kind: 0 applyBCscalarDensN, 1 applyBCscalarFieldNzero,
+ *       2 applyBCscalarDensC, 3 applyBCscalarFieldC */
+void OSFX(oracle_bc_scalar)(const oracle_grid *grd, FPinterp *a, int kind, int PX,
+                            int PY, int PZ) {
+  if (kind <= 1)
+    bc_scalar(a, grd->nxn, grd->nyn, grd->nzn, grd->nyn, 1, kind == 1, PX, PY, PZ);
+  else
+    bc_scalar(a, grd->nxc, grd->nyc, grd->nzc, grd->nyn, 0, 0, PX, PY, PZ);
+}
+
+/* interpN2Cinterp (src/Grid.cpp:164-185): interior cells from their eight
+ * nodes, summed in the reference's order. */
+static void interp_n2c(const oracle_grid *grd, FPinterp *c, const FPinterp *n) {
+  const int nyn = grd->nyn, nzn = grd->nzn, nyc = grd->nyc, nzc = grd->nzc;
+#define N(i, j, k) n[IDX3(i, j, k, nyn, nzn)]
+  for (int i = 1; i < grd->nxc - 1; i++)
+    for (int j = 1; j < nyc - 1; j++)
+      for (int k = 1; k < nzc - 1; k++)
+        c[IDX3(i, j, k, nyc, nzc)] =
+            .125 * (N(i, j, k) + N(i + 1, j, k) + N(i, j + 1, k) + N(i, j, k + 1) +
+                    N(i + 1, j + 1, k) + N(i + 1, j, k + 1) + N(i, j + 1, k + 1) +
+                    N(i + 1, j + 1, k + 1));
+#undef N
+}
+
+/* interpC2Ninterp (src/Grid.cpp:122-141): interior nodes from their eight
+ * cells. */
+static void interp_c2n(const oracle_grid *grd, FPinterp *n, const FPinterp *c) {
+  const int nyn = grd->nyn, nzn = grd->nzn, nyc = grd->nyc, nzc = grd->nzc;
+#define C(i, j, k) c[IDX3(i, j, k, nyc, nzc)]
+  for (int i = 1; i < grd->nxn - 1; i++)
+    for (int j = 1; j < nyn - 1; j++)
+      for (int k = 1; k < nzn - 1; k++)
+        n[IDX3(i, j, k, nyn, nzn)] =
+            .125 * (C(i, j, k) + C(i - 1, j, k) + C(i, j - 1, k) + C(i, j, k - 1) +
+                    C(i - 1, j - 1, k) + C(i - 1, j, k - 1) + C(i, j - 1, k - 1) +
+                    C(i - 1, j - 1, k - 1));
+#undef C
+}
+
+/* divN2C (src/Grid.cpp:267-319): interior cells. */
+static void div_n2c(const oracle_grid *grd, FPfield *d, const FPfield *X, const FPfield *Y,
+                    const FPfield *Z) {
+  const int nyn = grd->nyn, nzn = grd->nzn, nyc = grd->nyc, nzc = grd->nzc;
+  const FPfield invdx = (FPfield)grd->invdx, invdy = (FPfield)grd->invdy,
+                invdz = (FPfield)grd->invdz;
+#define V(a, i, j, k) a[IDX3(i, j, k, nyn, nzn)]
+  for (int i = 1; i < grd->nxc - 1; i++)
+    for (int j = 1; j < nyc - 1; j++)
+      for (int k = 1; k < nzc - 1; k++) {
+        FPfield cx = .25 * (V(X, i + 1, j, k) - V(X, i, j, k)) * invdx +
+                     .25 * (V(X, i + 1, j, k + 1) - V(X, i, j, k + 1)) * invdx +
+                     .25 * (V(X, i + 1, j + 1, k) - V(X, i, j + 1, k)) * invdx +
+                     .25 * (V(X, i + 1, j + 1, k + 1) - V(X, i, j + 1, k + 1)) * invdx;
+        FPfield cy = .25 * (V(Y, i, j + 1, k) - V(Y, i, j, k)) * invdy +
+                     .25 * (V(Y, i, j + 1, k + 1) - V(Y, i, j, k + 1)) * invdy +
+                     .25 * (V(Y, i + 1, j + 1, k) - V(Y, i + 1, j, k)) * invdy +
+                     .25 * (V(Y, i + 1, j + 1, k + 1) - V(Y, i + 1, j, k + 1)) * invdy;
+        FPfield cz = .25 * (V(Z, i, j, k + 1) - V(Z, i, j, k)) * invdz +
+                     .25 * (V(Z, i + 1, j, k + 1) - V(Z, i + 1, j, k)) * invdz +
+                     .25 * (V(Z, i, j + 1, k + 1) - V(Z, i, j + 1, k)) * invdz +
+                     .25 * (V(Z, i + 1, j + 1, k + 1) - V(Z, i + 1, j + 1, k)) * invdz;
+        d[IDX3(i, j, k, nyc, nzc)] = cx + cy + cz;
+      }
+#undef V
+}
+
+/* The driver's step between sumOverSpecies and calculateHatDensities
+ * (src/sputniPIC_CPU.cpp:276-277): ghost nodes of the net rhon
+ * (applyBCscalarDensN, in place), then rhoc = interpN2Cinterp(rhon) on the
+ * interior cells. */
+void OSFX(oracle_rhoc_from_rhon)(const oracle_grid *grd, FPinterp *rhoc, FPinterp *rhon,
+                                 int PX, int PY, int PZ) {
+  OSFX(oracle_bc_scalar)(grd, rhon, 0, PX, PY, PZ);
+  interp_n2c(grd, rhoc, rhon);
+}
+
+/* calculateHatDensities (src/MaxwellSolver.cpp:465-604).  ids: ns species'
+ * node moments (Jx Jy Jz and the six pressure components are read);
+ * B: Bxn Byn Bzn; rhoc: net cell density.  Writes rhoh (cells) and
+ * Jxh Jyh Jzh (nodes). */
+void OSFX(oracle_hat_densities)(const oracle_grid *grd, const oracle_moments *ids, int ns,
+                                const double *qom, const FPfield *Bxn, const FPfield *Byn,
+                                const FPfield *Bzn, const FPinterp *rhoc, double dt,
+                                double c, double th, int PX, int PY, int PZ,
+                                FPinterp *rhoh, FPinterp *Jxh, FPinterp *Jyh,
+                                FPinterp *Jzh) {
+  const int nxn = grd->nxn, nyn = grd->nyn, nzn = grd->nzn;
+  const int nxc = grd->nxc, nyc = grd->nyc, nzc = grd->nzc;
+  const long nn = (long)nxn * nyn * nzn, nc = (long)nxc * nyc * nzc;
+  const FPinterp invdx = (FPinterp)grd->invdx, invdy = (FPinterp)grd->invdy,
+                 invdz = (FPinterp)grd->invdz;
+  FPinterp *tXC = calloc(nc, sizeof(FPinterp)), *tYC = calloc(nc, sizeof(FPinterp)),
+           *tZC = calloc(nc, sizeof(FPinterp));
+  FPinterp *tXN = calloc(nn, sizeof(FPinterp)), *tYN = calloc(nn, sizeof(FPinterp)),
+           *tZN = calloc(nn, sizeof(FPinterp));
+  for (long n = 0; n < nn; n++) Jxh[n] = Jyh[n] = Jzh[n] = 0.0;
+  for (long n = 0; n < nc; n++) rhoh[n] = 0.0;
+  OSFX(oracle_bc_scalar)(grd, rhoh, 2, PX, PY, PZ);
+  for (int is = 0; is < ns; is++) {
+    const oracle_moments *m = &ids[is];
+    /* divSymmTensorN2C (src/Grid.cpp:322-398) */
+#define P(a, i, j, k) m->a[IDX3(i, j, k, nyn, nzn)]
+#define D4(a, i0, j0, k0, i1, j1, k1, inv)                                     \
+  (.25 * (P(a, i0 + i1, j0 + j1, k0 + k1) - P(a, i0, j0, k0)) * inv)
+    for (int i = 1; i < nxc - 1; i++)
+      for (int j = 1; j < nyc - 1; j++)
+        for (int k = 1; k < nzc - 1; k++) {
+          FPinterp c1X, c2X, c3X, c1Y, c2Y, c3Y, c1Z, c2Z, c3Z;
+          c1X = .25 * (P(pxx, i + 1, j, k) - P(pxx, i, j, k)) * invdx +
+                .25 * (P(pxx, i + 1, j, k + 1) - P(pxx, i, j, k + 1)) * invdx +
+                .25 * (P(pxx, i + 1, j + 1, k) - P(pxx, i, j + 1, k)) * invdx +
+                .25 * (P(pxx, i + 1, j + 1, k + 1) - P(pxx, i, j + 1, k + 1)) * invdx;
+          c2X = .25 * (P(pxy, i + 1, j, k) - P(pxy, i, j, k)) * invdx +
+                .25 * (P(pxy, i + 1, j, k + 1) - P(pxy, i, j, k + 1)) * invdx +
+                .25 * (P(pxy, i + 1, j + 1, k) - P(pxy, i, j + 1, k)) * invdx +
+                .25 * (P(pxy, i + 1, j + 1, k + 1) - P(pxy, i, j + 1, k + 1)) * invdx;
+          c3X = .25 * (P(pxz, i + 1, j, k) - P(pxz, i, j, k)) * invdx +
+                .25 * (P(pxz, i + 1, j, k + 1) - P(pxz, i, j, k + 1)) * invdx +
+                .25 * (P(pxz, i + 1, j + 1, k) - P(pxz, i, j + 1, k)) * invdx +
+                .25 * (P(pxz, i + 1, j + 1, k + 1) - P(pxz, i, j + 1, k + 1)) * invdx;
+          c1Y = .25 * (P(pxy, i, j + 1, k) - P(pxy, i, j, k)) * invdy +
+                .25 * (P(pxy, i, j + 1, k + 1) - P(pxy, i, j, k + 1)) * invdy +
+                .25 * (P(pxy, i + 1, j + 1, k) - P(pxy, i + 1, j, k)) * invdy +
+                .25 * (P(pxy, i + 1, j + 1, k + 1) - P(pxy, i + 1, j, k + 1)) * invdy;
+          c2Y = .25 * (P(pyy, i, j + 1, k) - P(pyy, i, j, k)) * invdy +
+                .25 * (P(pyy, i, j + 1, k + 1) - P(pyy, i, j, k + 1)) * invdy +
+                .25 * (P(pyy, i + 1, j + 1, k) - P(pyy, i + 1, j, k)) * invdy +
+                .25 * (P(pyy, i + 1, j + 1, k + 1) - P(pyy, i + 1, j, k + 1)) * invdy;
+          c3Y = .25 * (P(pyz, i, j + 1, k) - P(pyz, i, j, k)) * invdy +
+                .25 * (P(pyz, i, j + 1, k + 1) - P(pyz, i, j, k + 1)) * invdy +
+                .25 * (P(pyz, i + 1, j + 1, k) - P(pyz, i + 1, j, k)) * invdy +
+                .25 * (P(pyz, i + 1, j + 1, k + 1) - P(pyz, i + 1, j, k + 1)) * invdy;
+          c1Z = .25 * (P(pxz, i, j, k + 1) - P(pxz, i, j, k)) * invdz +
+                .25 * (P(pxz, i + 1, j, k + 1) - P(pxz, i + 1, j, k)) * invdz +
+                .25 * (P(pxz, i, j + 1, k + 1) - P(pxz, i, j + 1, k)) * invdz +
+                .25 * (P(pxz, i + 1, j + 1, k + 1) - P(pxz, i + 1, j + 1, k)) * invdz;
+          c2Z = .25 * (P(pyz, i, j, k + 1) - P(pyz, i, j, k)) * invdz +
+                .25 * (P(pyz, i + 1, j, k + 1) - P(pyz, i + 1, j, k)) * invdz +
+                .25 * (P(pyz, i, j + 1, k + 1) - P(pyz, i, j + 1, k)) * invdz +
+                .25 * (P(pyz, i + 1, j + 1, k + 1) - P(pyz, i + 1, j + 1, k)) * invdz;
+          c3Z = .25 * (P(pzz, i, j, k + 1) - P(pzz, i, j, k)) * invdz +
+                .25 * (P(pzz, i + 1, j, k + 1) - P(pzz, i + 1, j, k)) * invdz +
+                .25 * (P(pzz, i, j + 1, k + 1) - P(pzz, i, j + 1, k)) * invdz +
+                .25 * (P(pzz, i + 1, j + 1, k + 1) - P(pzz, i + 1, j + 1, k)) * invdz;
+          tXC[IDX3(i, j, k, nyc, nzc)] = c1X + c1Y + c1Z;
+          tYC[IDX3(i, j, k, nyc, nzc)] = c2X + c2Y + c2Z;
+          tZC[IDX3(i, j, k, nyc, nzc)] = c3X + c3Y + c3Z;
+        }
+#undef D4
+#undef P
+    /* scale (src/Basic.cpp:135-141) by -dt/2, FPfield factor */
+    const FPfield alfa = (FPfield)(-dt / 2.0);
+    for (long n = 0; n < nc; n++) {
+      tXC[n] *= alfa;
+      tYC[n] *= alfa;
+      tZC[n] *= alfa;
+    }
+    OSFX(oracle_bc_scalar)(grd, tXC, 2, PX, PY, PZ);
+    OSFX(oracle_bc_scalar)(grd, tYC, 2, PX, PY, PZ);
+    OSFX(oracle_bc_scalar)(grd, tZC, 2, PX, PY, PZ);
+    interp_c2n(grd, tXN, tXC);
+    interp_c2n(grd, tYN, tYC);
+    interp_c2n(grd, tZN, tZC);
+    for (long n = 0; n < nn; n++) {
+      tXN[n] += m->Jx[n];
+      tYN[n] += m->Jy[n];
+      tZN[n] += m->Jz[n];
+    }
+    /* PIdot */
+    const FPfield beta = (FPfield)(.5 * qom[is] * dt / c);
+    for (long n = 0; n < nn; n++) {
+      const FPfield omcx = beta * Bxn[n], omcy = beta * Byn[n], omcz = beta * Bzn[n];
+      const FPfield edotb = tXN[n] * omcx + tYN[n] * omcy + tZN[n] * omcz;
+      const FPfield denom = (FPfield)(1 / (1.0 + omcx * omcx + omcy * omcy + omcz * omcz));
+      Jxh[n] += (tXN[n] + (tYN[n] * omcz - tZN[n] * omcy + edotb * omcx)) * denom;
+      Jyh[n] += (tYN[n] + (tZN[n] * omcx - tXN[n] * omcz + edotb * omcy)) * denom;
+      Jzh[n] += (tZN[n] + (tXN[n] * omcy - tYN[n] * omcx + edotb * omcz)) * denom;
+    }
+  }
+  for (long n = 0; n < nc; n++) tXC[n] = 0.0;
+  OSFX(oracle_bc_scalar)(grd, Jxh, 0, PX, PY, PZ);
+  OSFX(oracle_bc_scalar)(grd, Jyh, 0, PX, PY, PZ);
+  OSFX(oracle_bc_scalar)(grd, Jzh, 0, PX, PY, PZ);
+  div_n2c(grd, tXC, Jxh, Jyh, Jzh);
+  for (long n = 0; n < nc; n++) rhoh[n] = rhoc[n] - dt * th * tXC[n];
+  OSFX(oracle_bc_scalar)(grd, rhoh, 2, PX, PY, PZ);
+  free(tXC); free(tYC); free(tZC);
+  free(tXN); free(tYN); free(tZN);
+}
+
+/* MaxwellImage (src/MaxwellSolver.cpp:95-257): the implicit field
+ * operator applied to a Krylov vector, vec and im holding the interior
+ * nodes' (x, y, z) components interleaved, i slowest (solver2phys and
+ * phys2solver, src/Basic.cpp:365-392). */
+void OSFX(oracle_maxwell_image)(const oracle_grid *grd, const FPfield *Bxn,
+                                const FPfield *Byn, const FPfield *Bzn,
+                                const oracle_moments *ids, int ns, const double *qom,
+                                double dt, double c, double th, double fourpi, int PX,
+                                int PY, int PZ, const FPfield *vec, FPfield *im) {
+  const int nxn = grd->nxn, nyn = grd->nyn, nzn = grd->nzn;
+  const int nxc = grd->nxc, nyc = grd->nyc, nzc = grd->nzc;
+  const long nn = (long)nxn * nyn * nzn, nc = (long)nxc * nyc * nzc;
+  const FPfield invdx = (FPfield)grd->invdx, invdy = (FPfield)grd->invdy,
+                invdz = (FPfield)grd->invdz;
+  const FPfield delt = (FPfield)(c * th * dt);
+  FPfield *v[3], *img[3], *D[3], *tmp[3];
+  for (int a = 0; a < 3; a++) {
+    v[a] = calloc(nn, sizeof(FPfield));
+    img[a] = calloc(nn, sizeof(FPfield));
+    D[a] = calloc(nn, sizeof(FPfield));
+    tmp[a] = calloc(nn, sizeof(FPfield));
+  }
+  FPfield *divC = calloc(nc, sizeof(FPfield));
+  const FPfield *p = vec;
+  for (int i = 1; i < nxn - 1; i++)
+    for (int j = 1; j < nyn - 1; j++)
+      for (int k = 1; k < nzn - 1; k++) {
+        const long n = IDX3(i, j, k, nyn, nzn);
+        v[0][n] = *p++;
+        v[1][n] = *p++;
+        v[2][n] = *p++;
+      }
+  for (int a = 0; a < 3; a++) OSFX(oracle_bc_scalar)(grd, (FPinterp *)v[a], 1, PX, PY, PZ);
+  /* lapN2N_V (src/Grid.cpp:668-717), then neg (src/Basic.cpp:281-286) */
+  for (int a = 0; a < 3; a++) {
+    const FPfield *s = v[a];
+    for (int i = 1; i < nxn - 1; i++)
+      for (int j = 1; j < nyn - 1; j++)
+        for (int k = 1; k < nzn - 1; k++) {
+          const long n = IDX3(i, j, k, nyn, nzn);
+          img[a][n] = (s[IDX3(i - 1, j, k, nyn, nzn)] - 2 * s[n] +
+                       s[IDX3(i + 1, j, k, nyn, nzn)]) * invdx * invdx +
+                      (s[IDX3(i, j - 1, k, nyn, nzn)] - 2 * s[n] +
+                       s[IDX3(i, j + 1, k, nyn, nzn)]) * invdy * invdy +
+                      (s[IDX3(i, j, k - 1, nyn, nzn)] - 2 * s[n] +
+                       s[IDX3(i, j, k + 1, nyn, nzn)]) * invdz * invdz;
+        }
+    for (long n = 0; n < nn; n++) img[a][n] = -img[a][n];
+  }
+  for (int is = 0; is < ns; is++) {
+    const FPfield beta = (FPfield)(.5 * qom[is] * dt / c);
+    const FPinterp *rhon = ids[is].rhon;
+    for (long n = 0; n < nn; n++) {
+      const FPfield omcx = beta * Bxn[n], omcy = beta * Byn[n], omcz = beta * Bzn[n];
+      const FPfield edotb = v[0][n] * omcx + v[1][n] * omcy + v[2][n] * omcz;
+      const FPfield denom = (FPfield)(fourpi / 2 * delt * dt / c * qom[is] * rhon[n] /
+                                      (1.0 + omcx * omcx + omcy * omcy + omcz * omcz));
+      D[0][n] += (v[0][n] + (v[1][n] * omcz - v[2][n] * omcy + edotb * omcx)) * denom;
+      D[1][n] += (v[1][n] + (v[2][n] * omcx - v[0][n] * omcz + edotb * omcy)) * denom;
+      D[2][n] += (v[2][n] + (v[0][n] * omcy - v[1][n] * omcx + edotb * omcz)) * denom;
+    }
+  }
+  for (int a = 0; a < 3; a++) OSFX(oracle_bc_scalar)(grd, (FPinterp *)D[a], 1, PX, PY, PZ);
+  div_n2c(grd, divC, D[0], D[1], D[2]);
+  OSFX(oracle_bc_scalar)(grd, (FPinterp *)divC, 3, PX, PY, PZ);
+  /* gradC2N (src/Grid.cpp:187-225) */
+#define S(i, j, k) divC[IDX3(i, j, k, nyc, nzc)]
+  for (int i = 1; i < nxn - 1; i++)
+    for (int j = 1; j < nyn - 1; j++)
+      for (int k = 1; k < nzn - 1; k++) {
+        const long n = IDX3(i, j, k, nyn, nzn);
+        tmp[0][n] = .25 * (S(i, j, k) - S(i - 1, j, k)) * invdx +
+                    .25 * (S(i, j, k - 1) - S(i - 1, j, k - 1)) * invdx +
+                    .25 * (S(i, j - 1, k) - S(i - 1, j - 1, k)) * invdx +
+                    .25 * (S(i, j - 1, k - 1) - S(i - 1, j - 1, k - 1)) * invdx;
+        tmp[1][n] = .25 * (S(i, j, k) - S(i, j - 1, k)) * invdy +
+                    .25 * (S(i, j, k - 1) - S(i, j - 1, k - 1)) * invdy +
+                    .25 * (S(i - 1, j, k) - S(i - 1, j - 1, k)) * invdy +
+                    .25 * (S(i - 1, j, k - 1) - S(i - 1, j - 1, k - 1)) * invdy;
+        tmp[2][n] = .25 * (S(i, j, k) - S(i, j, k - 1)) * invdz +
+                    .25 * (S(i - 1, j, k) - S(i - 1, j, k - 1)) * invdz +
+                    .25 * (S(i, j - 1, k) - S(i, j - 1, k - 1)) * invdz +
+                    .25 * (S(i - 1, j - 1, k) - S(i - 1, j - 1, k - 1)) * invdz;
+      }
+#undef S
+  /* sub, scale by delt*delt, sum D, sum vect (src/Basic.cpp) */
+  const FPfield dd = delt * delt;
+  for (int a = 0; a < 3; a++)
+    for (long n = 0; n < nn; n++) {
+      img[a][n] -= tmp[a][n];
+      img[a][n] *= dd;
+      img[a][n] += D[a][n];
+      img[a][n] += v[a][n];
+    }
+  FPfield *q = im;
+  for (int i = 1; i < nxn - 1; i++)
+    for (int j = 1; j < nyn - 1; j++)
+      for (int k = 1; k < nzn - 1; k++) {
+        const long n = IDX3(i, j, k, nyn, nzn);
+        *q++ = img[0][n];
+        *q++ = img[1][n];
+        *q++ = img[2][n];
+      }
+  for (int a = 0; a < 3; a++) {
+    free(v[a]); free(img[a]); free(D[a]); free(tmp[a]);
+  }
+  free(divC);
 }
 
 /* Wall-clock helper for the CPU baseline (omp_get_wtime when OpenMP is

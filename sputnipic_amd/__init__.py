@@ -8,6 +8,8 @@ reference's interface over that ABI.  There is no CPU fallback.
 """
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
 from .core import (Batch, DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams, Graph,  # noqa: F401
+                   InterpDensAux, MaxwellImage, SolveParams, applyBCscalar, calculateHatDensities,
+                   interpN2Crho_net,
                    GemSpecies, Grid, InterpDensNet, InterpDensSpecies, Parameters,
                    Particles, Stream, FIELD_NAMES, MOMENT_NAMES, PARTICLE_NAMES, applyBCids,
                    applyBCids_all, beam_species, check_domain, device_count, gem_fields, gem_particles, get_async, harris_species,

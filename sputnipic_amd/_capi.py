@@ -67,6 +67,16 @@ class spic_push(C.Structure):
                 ("PERIODICX", C.c_int), ("PERIODICY", C.c_int), ("PERIODICZ", C.c_int)]
 
 
+class spic_solve_params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("c", C.c_double), ("th", C.c_double),
+                ("fourpi", C.c_double), ("PERIODICX", C.c_int), ("PERIODICY", C.c_int),
+                ("PERIODICZ", C.c_int), ("qom", C.c_double * 8)]
+
+
+class spic_hat(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rhoh", "Jxh", "Jyh", "Jzh")]
+
+
 class spic_gem_params(C.Structure):
     _fields_ = [("B0x", C.c_double), ("B0y", C.c_double), ("B0z", C.c_double),
                 ("delta", C.c_double), ("pertX", C.c_double), ("fourpi", C.c_double)]
@@ -116,6 +126,19 @@ SIGNATURES = {
                                     C.POINTER(spic_push), _P]),
     "spic_sum_species": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments),
                                    C.POINTER(spic_moments), C.c_int, _P]),
+    "spic_field_workspace_bytes": (C.c_size_t, [C.POINTER(spic_grid), C.c_int]),
+    "spic_bc_scalar": (C.c_int, [C.POINTER(spic_grid), _P, C.c_int,
+                                 C.POINTER(spic_solve_params), _P]),
+    "spic_rhoc_from_rhon": (C.c_int, [C.POINTER(spic_grid), _P, _P,
+                                      C.POINTER(spic_solve_params), _P]),
+    "spic_hat_densities": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_field),
+                                     C.POINTER(spic_moments), C.c_int, _P,
+                                     C.POINTER(spic_solve_params), C.POINTER(spic_hat), _P,
+                                     C.c_size_t, _P]),
+    "spic_maxwell_image": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_field),
+                                     C.POINTER(spic_moments), C.c_int,
+                                     C.POINTER(spic_solve_params), _P, _P, _P, C.c_size_t,
+                                     _P]),
     "spic_engine_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_grid), C.c_int,
                                      C.POINTER(spic_engine_opts)]),
     "spic_engine_destroy": (C.c_int, [_P]),

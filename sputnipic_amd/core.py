@@ -227,8 +227,115 @@ class InterpDensSpecies:
 
 
 class InterpDensNet(InterpDensSpecies):
-    """The net node moments summed over species (include/InterpDensNet.h:9-40,
-    the ten node moments), FP64 on the device."""
+    """The net moments summed over species (include/InterpDensNet.h:9-40):
+    the ten node moments and the cell-centred rhoc, FP64 on the device."""
+
+    def __init__(self, grd: Grid):
+        super().__init__(grd)
+        ncell = grd.nxc * grd.nyc * grd.nzc
+        self.rhoc = DeviceArray(ncell, np.float64, fill_zero=True)
+
+
+class InterpDensAux:
+    """struct interpDens_aux (include/InterpDens_aux.h:6-13): rho hat on the
+    cells, J hat on the nodes, FP64 on the device."""
+
+    def __init__(self, grd: Grid):
+        self.grd = grd
+        ncell = grd.nxc * grd.nyc * grd.nzc
+        self.rhoh = DeviceArray(ncell, np.float64, fill_zero=True)
+        self.J = {n: DeviceArray(grd.nn, np.float64, fill_zero=True)
+                  for n in ("Jxh", "Jyh", "Jzh")}
+
+    def download(self) -> Dict[str, np.ndarray]:
+        out = {n: a.download() for n, a in self.J.items()}
+        out["rhoh"] = self.rhoh.download()
+        return out
+
+    def c_struct(self):
+        from ._capi import spic_hat
+        D = C.POINTER(C.c_double)
+        return spic_hat(C.cast(self.rhoh.ptr, D), *[C.cast(self.J[n].ptr, D)
+                                                    for n in ("Jxh", "Jyh", "Jzh")])
+
+
+@dataclass
+class SolveParams:
+    """The members of struct parameters the field-solve operators read
+    (include/Parameters.h): dt, c, th, fourpi, periodicity, qom per species;
+    defaults as src/RW_IO.cpp and inputfiles/GEM_3D.inp."""
+    dt: float = 0.25
+    c: float = 1.0
+    th: float = 1.0
+    fourpi: float = 12.5663706144
+    PERIODICX: bool = True
+    PERIODICY: bool = False
+    PERIODICZ: bool = True
+    qom: Sequence[float] = (-64.0, 1.0, -64.0, 1.0)
+
+    def c_struct(self):
+        from ._capi import spic_solve_params
+        q = (C.c_double * 8)(*[float(v) for v in self.qom][:8])
+        return spic_solve_params(self.dt, self.c, self.th, self.fourpi, int(self.PERIODICX),
+                                 int(self.PERIODICY), int(self.PERIODICZ), q)
+
+
+_WORK: Dict[tuple, "DeviceArray"] = {}
+
+
+def _workspace(grd: Grid, ns: int) -> "DeviceArray":
+    g = grd.c_struct()
+    need = int(lib().spic_field_workspace_bytes(C.byref(g), ns))
+    key = (grd.nxn, grd.nyn, grd.nzn)
+    w = _WORK.get(key)
+    if w is None or w.n * 8 < need:
+        w = DeviceArray(max(need // 8, 1), np.float64)
+        _WORK[key] = w
+    return w
+
+
+def applyBCscalar(a: "DeviceArray", grd: Grid, param: SolveParams, kind: int) -> None:
+    """Ghost update in place: kind 0 applyBCscalarDensN, 1
+    applyBCscalarFieldNzero, 2 applyBCscalarDensC, 3 applyBCscalarFieldC
+    (src/BC.cpp)."""
+    g, p = grd.c_struct(), param.c_struct()
+    check(lib().spic_bc_scalar(C.byref(g), a.ptr, kind, C.byref(p), None), "spic_bc_scalar")
+
+
+def interpN2Crho_net(idn: InterpDensNet, grd: Grid, param: SolveParams) -> None:
+    """applyBCscalarDensN(idn.rhon) then interpN2Cinterp(idn.rhoc, idn.rhon),
+    the driver's step before the solver (src/sputniPIC_CPU.cpp:276-277)."""
+    g, p = grd.c_struct(), param.c_struct()
+    check(lib().spic_rhoc_from_rhon(C.byref(g), idn.d["rhon"].ptr, idn.rhoc.ptr, C.byref(p),
+                                    None), "spic_rhoc_from_rhon")
+
+
+def calculateHatDensities(id_aux: InterpDensAux, idn: InterpDensNet,
+                          ids: Sequence[InterpDensSpecies], field: "EMField", grd: Grid,
+                          param: SolveParams, stream: Optional["Stream"] = None) -> None:
+    """calculateHatDensities (src/MaxwellSolver.cpp:465-604), same argument
+    order; ns = len(ids)."""
+    ns = len(ids)
+    g, f, p, h = grd.c_struct(), field.c_struct(), param.c_struct(), id_aux.c_struct()
+    moms = (spic_moments * ns)(*[m.c_struct() for m in ids])
+    w = _workspace(grd, ns)
+    check(lib().spic_hat_densities(C.byref(g), C.byref(f), moms, ns, idn.rhoc.ptr, C.byref(p),
+                                   C.byref(h), w.ptr, w.n * 8, stream.ptr if stream else None),
+          "spic_hat_densities")
+
+
+def MaxwellImage(im: "DeviceArray", vector: "DeviceArray", field: "EMField",
+                 ids: Sequence[InterpDensSpecies], grd: Grid, param: SolveParams,
+                 stream: Optional["Stream"] = None) -> None:
+    """MaxwellImage (src/MaxwellSolver.cpp:95-257), same argument order:
+    im = A vector over the 3 (nxn-2)(nyn-2)(nzn-2) interior unknowns."""
+    ns = len(ids)
+    g, f, p = grd.c_struct(), field.c_struct(), param.c_struct()
+    moms = (spic_moments * max(ns, 1))(*[m.c_struct() for m in ids])
+    w = _workspace(grd, ns)
+    check(lib().spic_maxwell_image(C.byref(g), C.byref(f), moms, ns, C.byref(p), vector.ptr,
+                                   im.ptr, w.ptr, w.n * 8, stream.ptr if stream else None),
+          "spic_maxwell_image")
 
 
 def _check_prec(*objs):
