@@ -487,6 +487,91 @@ def run_batch(steps: int, warmup: int) -> dict:
     }
 
 
+def field_bytes(grd, ns: int):
+    """Algorithmic bytes of the field-solve operators on grd (FP64): the
+    net rhoc step + hat densities read the net rhon, every species' J and
+    pressure (9 node arrays), B and rhoc and write rhoc, J hat and rho hat;
+    MaxwellImage reads the Krylov vector, B and every species' rhon and
+    writes the image."""
+    nn = grd.nn
+    nc = grd.nxc * grd.nyc * grd.nzc
+    m = (grd.nxn - 2) * (grd.nyn - 2) * (grd.nzn - 2)
+    hat = 8 * ((7 + 9 * ns) * nn + 3 * nc)
+    image = 8 * (6 * m + (3 + ns) * nn)
+    return hat, image
+
+
+def run_field(reps: int, cpu: bool) -> dict:
+    """The field-solve consumers of the deposit on the 128x64x64 grid, FP64:
+    the driver's net-rhoc step with calculateHatDensities, and MaxwellImage
+    (one GMRES operator application), timed with HIP events over `reps`
+    calls each; with `cpu`, the reference's own routines (oracle/_ref, -O3)
+    on the host cores at the same grid."""
+    import numpy as np
+
+    import sputnipic_amd as S
+
+    grd = S.Grid(*GRID, *BOX, "f64")
+    field = S.gem_fields(grd, S.GemParams())
+    rng = np.random.default_rng(1)
+    ids = [S.InterpDensSpecies(grd) for _ in range(NS)]
+    for m in ids:
+        m.upload({n: rng.normal(0, 1e-2, grd.nn) for n in S.MOMENT_NAMES})
+    net = S.InterpDensNet(grd)
+    S.sumOverSpecies(net, ids, grd, NS)
+    sp = S.SolveParams()
+    aux = S.InterpDensAux(grd)
+    nvec = 3 * (grd.nxn - 2) * (grd.nyn - 2) * (grd.nzn - 2)
+    vec = S.DeviceArray.from_numpy(rng.normal(0, 1e-2, nvec))
+    im = S.DeviceArray(nvec, np.float64)
+    stream = S.Stream()
+
+    def hat():
+        S.interpN2Crho_net(net, grd, sp)
+        S.calculateHatDensities(aux, net, ids, field, grd, sp, stream)
+
+    def image():
+        S.MaxwellImage(im, vec, field, ids, grd, sp, stream)
+
+    out = {"grid": "x".join(map(str, GRID)), "species": NS, "reps": reps}
+    hat_b, image_b = field_bytes(grd, NS)
+    S.set_async(True)
+    try:
+        for name, fn, nbytes in (("hat_densities", hat, hat_b), ("maxwell_image", image, image_b)):
+            fn()
+            stream.synchronize()
+            a, b = S.Event(), S.Event()
+            a.record(stream)
+            for _ in range(reps):
+                fn()
+            b.record(stream)
+            stream.synchronize()
+            ms = a.elapsed_ms(b) / reps
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out[name] = {"gpu_ms": ms, "algorithmic_bytes": nbytes, "achieved_GBs": gbs,
+                         "roofline_frac": gbs / HBM_PEAK_GBS}
+        S.check_domain(stream)
+    finally:
+        S.set_async(False)
+    if cpu:
+        import oracle as O
+        if O.ref_available("f64_O3"):
+            cfg = O.GemConfig(nxc=GRID[0], nyc=GRID[1], nzc=GRID[2], Lx=BOX[0], Ly=BOX[1],
+                              Lz=BOX[2], npcelx=[1] * NS, npcely=[1] * NS, npcelz=[1] * NS)
+            ref = O.RefState(cfg, "f64_O3")
+            creps = 3
+            th, ti = ref.time_field(creps)
+            out["hat_densities"]["cpu_reference_ms"] = 1e3 * th / creps
+            out["maxwell_image"]["cpu_reference_ms"] = 1e3 * ti / creps
+            out["cpu_threads"] = ref.max_threads()
+            ref.close()
+    log("field solve: " + ", ".join(
+        f"{k} {v['gpu_ms']:.3f} ms ({v['achieved_GBs']:.0f} GB/s"
+        + (f"; reference CPU {v['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in v else "")
+        + ")" for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
 def form_errors(m32, m64, interior: bool = False) -> dict:
     """FP32 form against FP64 form after the same cycles from the same
     initial condition: per moment, max |m32 - m64| over all species and
@@ -614,6 +699,8 @@ def main() -> int:
                          "species, 314.6M particles, FP32, 50 cycles in one HIP graph; "
                          "batch: 16 independent Harris-sheet instances in FP32, one "
                          "launch per kernel and phase, autotuned, in one HIP graph")
+    ap.add_argument("--no-field", action="store_true",
+                    help="skip the field-solve operators (hat densities, MaxwellImage)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once device to device (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -751,6 +838,10 @@ def main() -> int:
     if not args.no_cpu_baseline:
         with _StdoutToStderr():
             cpu = cpu_baseline(args.cpu_cycles)
+    field = None
+    if not args.no_field:
+        with _StdoutToStderr():
+            field = run_field(20, cpu=not args.no_cpu_baseline)
     line = {
         "metric": METRIC,
         "value": f["value"],
@@ -786,6 +877,7 @@ def main() -> int:
         },
         "forms": {k: {kk: vv for kk, vv in v.items()} for k, v in forms.items()},
         "cpu_baseline": cpu,
+        "field_solve": field,
     }
     if "f64" in forms and "f32" in forms:
         line["fp32_over_fp64"] = forms["f32"]["value"] / forms["f64"]["value"]

@@ -142,6 +142,11 @@ int spic_mover_interp(const spic_grid *grd, const spic_field *field,
 int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
                        spic_species *sp, void *stream);
 
+/* The same snapshot words for a species in the caller's order (its nop
+ * particles, element type per grd->prec). */
+int spic_snapshot_be32(const spic_grid *grd, const spic_species *sp,
+                       void *const dst[6], void *stream);
+
 /* setZeroSpeciesDensities, node-moment part: zero the ten arrays. */
 int spic_zero_moments(const spic_grid *grd, spic_moments *mom, void *stream);
 
@@ -289,6 +294,14 @@ int spic_engine_store(spic_engine *eng, int is, spic_species *sp,
 int spic_engine_phase(spic_engine *eng, const spic_field *field,
                       const spic_moments *mom, const spic_push *push,
                       void *stream);
+/* Particle snapshot of species `is` in load order: x y z u v w as
+ * big-endian IEEE float32 words (values rounded to float) into six device
+ * arrays of nop uint32 -- the datasets HDF5_Write_Particles writes per
+ * species as H5T_IEEE_F32BE (src/RW_IO.cpp:376-460).  One kernel; the
+ * caller copies the words out (spic_memcpy_d2h_async on a side stream lets
+ * the next phase run meanwhile). */
+int spic_engine_snapshot_be32(spic_engine *eng, int is, void *const dst[6],
+                              void *stream);
 /* Rebuild every species' tile segments now. */
 int spic_engine_sort(spic_engine *eng, void *stream);
 int spic_engine_get_stats(const spic_engine *eng, spic_engine_stats *st);
@@ -402,6 +415,13 @@ int spic_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int spic_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int spic_memset(void *dptr, int value, size_t bytes);
 int spic_synchronize(void);
+/* Pinned host memory (hipHostMalloc) and an asynchronous device-to-host
+ * copy, for snapshots copied out on a side stream. */
+int spic_host_alloc(void **hptr, size_t bytes);
+int spic_host_free(void *hptr);
+int spic_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream);
+/* Make `stream` wait for `event` (recorded on another stream). */
+int spic_stream_wait_event(void *stream, void *event);
 int spic_stream_create(void **stream);
 int spic_stream_destroy(void *stream);
 int spic_stream_synchronize(void *stream);

@@ -561,6 +561,8 @@ class RefState:
             L.ref_zero_species_moments.argtypes = [_P]
             L.ref_sum_species.argtypes = [_P, C.POINTER(C.c_double)]
             L.ref_hat_densities.argtypes = [_P, _P]
+            L.ref_time_field.argtypes = [_P, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
             L.ref_maxwell_image.argtypes = [_P, _P, _P]
             L.ref_time_cycles.argtypes = [_P, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
@@ -661,3 +663,10 @@ class RefState:
         im = np.zeros_like(vec)
         self.L.ref_maxwell_image(self.h, _ptr(vec), _ptr(im))
         return im
+
+    def time_field(self, reps: int = 1):
+        """Seconds for `reps` hat-density steps and `reps` MaxwellImage
+        calls (ref_time_field)."""
+        th, ti = C.c_double(), C.c_double()
+        self.L.ref_time_field(self.h, reps, C.byref(th), C.byref(ti))
+        return th.value, ti.value

@@ -354,5 +354,38 @@ double ref_time_cycles(void *h, int ncycles, double *t_mover, double *t_deposit)
   return tm + td;
 }
 
+// CPU timing of the field-solve side (src/sputniPIC_CPU.cpp:258-281):
+// `reps` times sumOverSpecies + applyBCscalarDensN + interpN2Cinterp +
+// calculateHatDensities, and `reps` calls of MaxwellImage on one vector.
+void ref_time_field(void *h, int reps, double *t_hat, double *t_image) {
+  ref_state *s = (ref_state *)h;
+  interpDensNet idn;
+  interp_dens_net_allocate(&s->grd, &idn);
+  interpDens_aux id_aux;
+  interp_dens_aux_allocate(&s->grd, &id_aux);
+  double t0 = omp_get_wtime();
+  for (int r = 0; r < reps; r++) {
+    setZeroNetDensities(&idn, &s->grd);
+    sumOverSpecies(&idn, s->ids, &s->grd, s->param.ns);
+    applyBCscalarDensN(idn.rhon, &s->grd, &s->param);
+    interpN2Cinterp(idn.rhoc, idn.rhon, &s->grd);
+    calculateHatDensities(&id_aux, &idn, s->ids, &s->field, &s->grd, &s->param);
+  }
+  double t1 = omp_get_wtime();
+  long n = 3L * (s->grd.nxn - 2) * (s->grd.nyn - 2) * (s->grd.nzn - 2);
+  FPfield *v = new FPfield[n];
+  FPfield *r = new FPfield[n];
+  for (long i = 0; i < n; i++) v[i] = (FPfield)(1e-3 * (double)(i % 97));
+  double t2 = omp_get_wtime();
+  for (int q = 0; q < reps; q++) MaxwellImage(r, v, &s->field, s->ids, &s->grd, &s->param);
+  double t3 = omp_get_wtime();
+  delete[] v;
+  delete[] r;
+  interp_dens_aux_deallocate(&s->grd, &id_aux);
+  interp_dens_net_deallocate(&s->grd, &idn);
+  if (t_hat) *t_hat = t1 - t0;
+  if (t_image) *t_image = t3 - t2;
+}
+
 int ref_max_threads(void) { return omp_get_max_threads(); }
 }  // extern "C"

@@ -147,6 +147,13 @@ SIGNATURES = {
     "spic_engine_phase": (C.c_int, [_P, C.POINTER(spic_field), C.POINTER(spic_moments),
                                     C.POINTER(spic_push), _P]),
     "spic_engine_sort": (C.c_int, [_P, _P]),
+    "spic_engine_snapshot_be32": (C.c_int, [_P, C.c_int, C.POINTER(_P), _P]),
+    "spic_snapshot_be32": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_species),
+                                     C.POINTER(_P), _P]),
+    "spic_host_alloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
+    "spic_host_free": (C.c_int, [_P]),
+    "spic_memcpy_d2h_async": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "spic_stream_wait_event": (C.c_int, [_P, _P]),
     "spic_engine_get_stats": (C.c_int, [_P, C.POINTER(spic_engine_stats)]),
     "spic_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(spic_batch_member), C.c_int,
                                     C.c_int, C.c_int]),

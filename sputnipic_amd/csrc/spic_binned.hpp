@@ -208,4 +208,11 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, T *const dst[7], hipStream_t s);
 
+// bin_store of x y z u v w only, as big-endian IEEE float32 words (values
+// rounded to float): the particle datasets of a snapshot.
+template <typename T>
+hipError_t bin_store_be32(const BinLayout &L, T *const src[7], const uint32_t *id,
+                          const int64_t *seg, const unsigned int *cnt,
+                          unsigned int ovf_cap, uint32_t *const dst[6], hipStream_t s);
+
 }  // namespace spic

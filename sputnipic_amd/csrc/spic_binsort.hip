@@ -104,6 +104,10 @@ struct Arrays7 {
   T *p[7];
 };
 
+struct Words6 {
+  uint32_t *p[6];
+};
+
 template <typename T>
 __global__ void k_bin_gather(Arrays7<T> src, const uint32_t *__restrict__ src_id,
                              const uint32_t *__restrict__ perm,
@@ -157,6 +161,28 @@ __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
     const uint32_t k = id[i];
 #pragma unroll
     for (int a = 0; a < 7; ++a) dst.p[a][k] = src.p[a][i];
+  }
+}
+
+// Big-endian IEEE float32 bits of v rounded to float (H5T_IEEE_F32BE).
+template <typename T>
+__device__ __forceinline__ uint32_t be32(T v) {
+  return __builtin_bswap32(__float_as_uint((float)v));
+}
+
+// bin_store's scatter to load order, writing x y z u v w as big-endian
+// float32: the datasets of a particle snapshot.
+template <typename T>
+__global__ void k_bin_store_be32(Arrays7<T> src, const uint32_t *__restrict__ id,
+                                 const int64_t *__restrict__ seg,
+                                 const unsigned int *__restrict__ cnt, unsigned int ovf_cap,
+                                 int ntiles, Words6 dst) {
+  const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
+  for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
+    const int64_t i = v.slot(r);
+    const uint32_t k = id[i];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) dst.p[a][k] = be32(src.p[a][i]);
   }
 }
 
@@ -427,6 +453,28 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      seg, cnt, ovf_cap, L.ntiles, ad);
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t bin_store_be32(const BinLayout &L, T *const src[7], const uint32_t *id,
+                          const int64_t *seg, const unsigned int *cnt, unsigned int ovf_cap,
+                          uint32_t *const dst[6], hipStream_t s) {
+  Arrays7<T> as;
+  Words6 ad;
+  for (int a = 0; a < 7; ++a) as.p[a] = src[a];
+  for (int a = 0; a < 6; ++a) ad.p[a] = dst[a];
+  hipLaunchKernelGGL(k_bin_store_be32<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id, seg,
+                     cnt, ovf_cap, L.ntiles, ad);
+  return hipGetLastError();
+}
+
+template hipError_t bin_store_be32<double>(const BinLayout &, double *const[7],
+                                           const uint32_t *, const int64_t *,
+                                           const unsigned int *, unsigned int,
+                                           uint32_t *const[6], hipStream_t);
+template hipError_t bin_store_be32<float>(const BinLayout &, float *const[7],
+                                          const uint32_t *, const int64_t *,
+                                          const unsigned int *, unsigned int,
+                                          uint32_t *const[6], hipStream_t);
 
 template hipError_t bin_store<double>(const BinLayout &, double *const[7],
                                       const uint32_t *, const int64_t *,

@@ -440,6 +440,29 @@ int spic_synchronize(void) {
   return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipDeviceSynchronize");
 }
 
+int spic_host_alloc(void **hptr, size_t bytes) {
+  if (!hptr) return fail(SPIC_EINVAL, "hptr is NULL");
+  *hptr = nullptr;
+  hipError_t e = hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipHostMalloc");
+}
+
+int spic_host_free(void *hptr) {
+  hipError_t e = hipHostFree(hptr);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipHostFree");
+}
+
+int spic_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipMemcpyAsync D2H");
+}
+
+int spic_stream_wait_event(void *stream, void *event) {
+  if (!event) return fail(SPIC_EINVAL, "event is NULL");
+  hipError_t e = hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0);
+  return e == hipSuccess ? SPIC_OK : hip_fail(e, "hipStreamWaitEvent");
+}
+
 int spic_stream_create(void **stream) {
   if (!stream) return fail(SPIC_EINVAL, "stream is NULL");
   hipStream_t s;

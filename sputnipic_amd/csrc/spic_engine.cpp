@@ -410,6 +410,27 @@ int spic_engine_store(spic_engine *e, int is, spic_species *dst, void *stream) {
   return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine store");
 }
 
+int spic_engine_snapshot_be32(spic_engine *e, int is, void *const dst[6], void *stream) {
+  if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  if (is < 0 || is >= e->ns || !e->sp[is].loaded)
+    return fail(SPIC_EINVAL, "species %d not loaded", is);
+  if (!dst) return fail(SPIC_EINVAL, "snapshot arrays are NULL");
+  for (int a = 0; a < 6; ++a)
+    if (!dst[a]) return fail(SPIC_EINVAL, "snapshot array %d is NULL", a);
+  std::lock_guard<std::mutex> lock(e->mu);
+  auto &sp = e->sp[is];
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t *const *d = (uint32_t *const *)dst;
+  hipError_t err =
+      e->esz == 8
+          ? spic::bin_store_be32<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                         sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, d, s)
+          : spic::bin_store_be32<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                        sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, d, s);
+  if (err == hipSuccess && !spic::async_mode()) err = hipStreamSynchronize(s);
+  return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine snapshot");
+}
+
 int spic_engine_sort(spic_engine *e, void *stream) {
   if (!e) return fail(SPIC_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock(e->mu);
