@@ -694,11 +694,12 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
-    ap.add_argument("--workload", choices=("gem", "stress", "batch"), default="gem",
+    ap.add_argument("--workload", choices=("gem", "stress", "batch", "field"), default="gem",
                     help="gem: the 262M-particle Harris sheet (default); stress: 8 "
                          "species, 314.6M particles, FP32, 50 cycles in one HIP graph; "
                          "batch: 16 independent Harris-sheet instances in FP32, one "
-                         "launch per kernel and phase, autotuned, in one HIP graph")
+                         "launch per kernel and phase, autotuned, in one HIP graph; "
+                         "field: only the field-solve operators on the 128x64x64 grid")
     ap.add_argument("--no-field", action="store_true",
                     help="skip the field-solve operators (hat densities, MaxwellImage)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -762,6 +763,12 @@ def main() -> int:
             "cpu_baseline": None,
         }
         print(json.dumps(line), flush=True)
+        return 0
+    if args.workload == "field":
+        with _StdoutToStderr():
+            f = run_field(args.steps, cpu=not args.no_cpu_baseline)
+        print(json.dumps({"metric": "field-solve operators, ms per call", "field_solve": f}),
+              flush=True)
         return 0
     if args.workload == "batch":
         f = run_batch(args.steps, args.warmup)

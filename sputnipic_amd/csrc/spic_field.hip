@@ -30,70 +30,86 @@ struct BcArrays {
   double *a[kMaxBcArrays];
 };
 
-// One workgroup per array runs the reference's ghost-update sequence:
-// periodic faces x, y, z; the edges along x, y, z; the eight corners; the
-// non-periodic faces x, y, z (copied from the adjacent plane, or zeroed).
-// A node array takes its ghosts from planes n-3 and 2, a cell array from
-// n-2 and 1; all four reference routines run the y-edge loop to the node
-// count (j < nyn - 1).  No step reads an element the same step writes (for
-// grids of at least one physical cell per axis), so each step's elements
-// are updated in parallel; a device fence and a barrier separate steps.
-__global__ __launch_bounds__(1024) void k_bc(BcArrays arrs, int n0, int n1, int n2, int nyn,
-                                             int nodes, int zero_faces, int px, int py,
-                                             int pz) {
-  SPIC_GLOBAL double *a = gmem(arrs.a[blockIdx.x]);
+// Ghost updates, the reference's sequence (periodic faces x, y, z; edges
+// along x, y, z; corners; non-periodic faces x, y, z, copied from the
+// adjacent plane or zeroed) in two launches with the same result:
+//  * k_bc_periodic: the periodic faces, edges and corners.  A node array
+//    takes its ghosts from planes n-3 and 2, a cell array from n-2 and 1.
+//    Every element these steps read is interior and no two steps write the
+//    same element, so they run together.  The reference runs the y-edge
+//    loop to the node count (j < nyn - 1), which for a cell array also
+//    writes the four corners at j = nyc - 1; the corner step, which runs
+//    whenever the y-edge step does, overwrites those, so they are skipped.
+//  * k_bc_faces: the non-periodic faces.  Applied in x, y, z order, the
+//    plane copies leave at every element with a ghost index on a
+//    non-periodic axis the value of the element with those indices moved
+//    one plane inward (clamped), read after the periodic steps; zeroed
+//    faces leave zero there.
+__global__ void k_bc_periodic(BcArrays arrs, int n0, int n1, int n2, int nyn, int nodes,
+                              int px, int py, int pz) {
+  SPIC_GLOBAL double *a = gmem(arrs.a[blockIdx.y]);
   const int lo = nodes ? 3 : 2, hi = nodes ? 2 : 1;
-  const int t = threadIdx.x, T = blockDim.x;
 #define A(i, j, k) a[((int64_t)(i) * n1 + (j)) * n2 + (k)]
-#define STEP_END   \
-  __threadfence(); \
-  __syncthreads();
-  if (px)
-    for (int e = t; e < (n1 - 2) * (n2 - 2); e += T) {
-      const int j = 1 + e / (n2 - 2), k = 1 + e % (n2 - 2);
+  const int64_t fx = px ? (int64_t)(n1 - 2) * (n2 - 2) : 0;
+  const int64_t fy = py ? (int64_t)(n0 - 2) * (n2 - 2) : 0;
+  const int64_t fz = pz ? (int64_t)(n0 - 2) * (n1 - 2) : 0;
+  const int64_t ex = (py || pz) ? n0 - 2 : 0;
+  // y-edge rows j = 1 .. nyn - 2, without the cell corner row j = n1 - 1
+  const int yend = nyn - 1 < n1 - 1 ? nyn - 1 : n1 - 1;
+  const int64_t ey = (px || pz) ? yend - 1 : 0;
+  const int64_t ez = (px || py) ? n2 - 2 : 0;
+  const int64_t co = (px || py || pz) ? 1 : 0;
+  const int64_t total = fx + fy + fz + ex + ey + ez + co;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = e;
+    if (r < fx) {
+      const int j = 1 + (int)(r / (n2 - 2)), k = 1 + (int)(r % (n2 - 2));
       A(0, j, k) = A(n0 - lo, j, k);
       A(n0 - 1, j, k) = A(hi, j, k);
+      continue;
     }
-  STEP_END
-  if (py)
-    for (int e = t; e < (n0 - 2) * (n2 - 2); e += T) {
-      const int i = 1 + e / (n2 - 2), k = 1 + e % (n2 - 2);
+    r -= fx;
+    if (r < fy) {
+      const int i = 1 + (int)(r / (n2 - 2)), k = 1 + (int)(r % (n2 - 2));
       A(i, 0, k) = A(i, n1 - lo, k);
       A(i, n1 - 1, k) = A(i, hi, k);
+      continue;
     }
-  STEP_END
-  if (pz)
-    for (int e = t; e < (n0 - 2) * (n1 - 2); e += T) {
-      const int i = 1 + e / (n1 - 2), j = 1 + e % (n1 - 2);
+    r -= fy;
+    if (r < fz) {
+      const int i = 1 + (int)(r / (n1 - 2)), j = 1 + (int)(r % (n1 - 2));
       A(i, j, 0) = A(i, j, n2 - lo);
       A(i, j, n2 - 1) = A(i, j, hi);
+      continue;
     }
-  STEP_END
-  if (py || pz)
-    for (int i = 1 + t; i < n0 - 1; i += T) {
+    r -= fz;
+    if (r < ex) {
+      const int i = 1 + (int)r;
       A(i, n1 - 1, n2 - 1) = A(i, hi, hi);
       A(i, 0, 0) = A(i, n1 - lo, n2 - lo);
       A(i, 0, n2 - 1) = A(i, n1 - lo, hi);
       A(i, n1 - 1, 0) = A(i, hi, n2 - lo);
+      continue;
     }
-  STEP_END
-  if (px || pz)
-    for (int j = 1 + t; j < nyn - 1; j += T) {
+    r -= ex;
+    if (r < ey) {
+      const int j = 1 + (int)r;
       A(n0 - 1, j, n2 - 1) = A(hi, j, hi);
       A(0, j, 0) = A(n0 - lo, j, n2 - lo);
       A(0, j, n2 - 1) = A(n0 - lo, j, hi);
       A(n0 - 1, j, 0) = A(hi, j, n2 - lo);
+      continue;
     }
-  STEP_END
-  if (px || py)
-    for (int k = 1 + t; k < n2 - 1; k += T) {
+    r -= ey;
+    if (r < ez) {
+      const int k = 1 + (int)r;
       A(n0 - 1, n1 - 1, k) = A(hi, hi, k);
       A(0, 0, k) = A(n0 - lo, n1 - lo, k);
       A(n0 - 1, 0, k) = A(hi, n1 - lo, k);
       A(0, n1 - 1, k) = A(n0 - lo, hi, k);
+      continue;
     }
-  STEP_END
-  if ((px || py || pz) && t == 0) {
     A(n0 - 1, n1 - 1, n2 - 1) = A(hi, hi, hi);
     A(0, n1 - 1, n2 - 1) = A(n0 - lo, hi, hi);
     A(n0 - 1, 0, n2 - 1) = A(hi, n1 - lo, hi);
@@ -103,28 +119,51 @@ __global__ __launch_bounds__(1024) void k_bc(BcArrays arrs, int n0, int n1, int 
     A(n0 - 1, 0, 0) = A(hi, n1 - lo, n2 - lo);
     A(0, 0, 0) = A(n0 - lo, n1 - lo, n2 - lo);
   }
-  STEP_END
-  if (!px)
-    for (int e = t; e < n1 * n2; e += T) {
-      const int j = e / n2, k = e % n2;
-      A(0, j, k) = zero_faces ? 0.0 : A(1, j, k);
-      A(n0 - 1, j, k) = zero_faces ? 0.0 : A(n0 - 2, j, k);
+}
+
+// Index moved one plane inward if it is a ghost of a non-periodic axis.
+__device__ __forceinline__ int clamp_ghost(int v, int n, int periodic) {
+  if (periodic) return v;
+  return v == 0 ? 1 : (v == n - 1 ? n - 2 : v);
+}
+
+// Every element on the ghost planes of the non-periodic axes (an element
+// on two such planes is visited twice and written the same value twice).
+__global__ void k_bc_faces(BcArrays arrs, int n0, int n1, int n2, int zero_faces, int px,
+                           int py, int pz) {
+  SPIC_GLOBAL double *a = gmem(arrs.a[blockIdx.y]);
+  const int64_t sx = px ? 0 : 2 * (int64_t)n1 * n2;
+  const int64_t sy = py ? 0 : 2 * (int64_t)n0 * n2;
+  const int64_t sz = pz ? 0 : 2 * (int64_t)n0 * n1;
+  const int64_t total = sx + sy + sz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    int64_t r = e;
+    if (r < sx) {
+      const int64_t plane = (int64_t)n1 * n2;
+      i = r < plane ? 0 : n0 - 1;
+      r %= plane;
+      j = (int)(r / n2);
+      k = (int)(r % n2);
+    } else if ((r -= sx) < sy) {
+      const int64_t plane = (int64_t)n0 * n2;
+      j = r < plane ? 0 : n1 - 1;
+      r %= plane;
+      i = (int)(r / n2);
+      k = (int)(r % n2);
+    } else {
+      r -= sy;
+      const int64_t plane = (int64_t)n0 * n1;
+      k = r < plane ? 0 : n2 - 1;
+      r %= plane;
+      i = (int)(r / n1);
+      j = (int)(r % n1);
     }
-  STEP_END
-  if (!py)
-    for (int e = t; e < n0 * n2; e += T) {
-      const int i = e / n2, k = e % n2;
-      A(i, 0, k) = zero_faces ? 0.0 : A(i, 1, k);
-      A(i, n1 - 1, k) = zero_faces ? 0.0 : A(i, n1 - 2, k);
-    }
-  STEP_END
-  if (!pz)
-    for (int e = t; e < n0 * n1; e += T) {
-      const int i = e / n1, j = e % n1;
-      A(i, j, 0) = zero_faces ? 0.0 : A(i, j, 1);
-      A(i, j, n2 - 1) = zero_faces ? 0.0 : A(i, j, n2 - 2);
-    }
-#undef STEP_END
+    A(i, j, k) = zero_faces ? 0.0
+                            : A(clamp_ghost(i, n0, px), clamp_ghost(j, n1, py),
+                                clamp_ghost(k, n2, pz));
+  }
 #undef A
 }
 
@@ -416,9 +455,17 @@ hipError_t launch_bc(double *const *arrs, int narr, const Dims &d, bool nodes,
   for (int i = 0; i < narr; ++i) b.a[i] = arrs[i];
   const int n0 = nodes ? d.nxn : d.nxc, n1 = nodes ? d.nyn : d.nyc,
             n2 = nodes ? d.nzn : d.nzc;
-  hipLaunchKernelGGL(k_bc, dim3(narr), dim3(1024), 0, s, b, n0, n1, n2, d.nyn,
-                     nodes ? 1 : 0, zero_faces ? 1 : 0, p->PERIODICX ? 1 : 0,
-                     p->PERIODICY ? 1 : 0, p->PERIODICZ ? 1 : 0);
+  const int px = p->PERIODICX ? 1 : 0, py = p->PERIODICY ? 1 : 0, pz = p->PERIODICZ ? 1 : 0;
+  // enough threads for the largest face pair; grid-stride loops do the rest
+  const int64_t faces = 2 * ((int64_t)n0 * n1 + (int64_t)n0 * n2 + (int64_t)n1 * n2);
+  unsigned blocks = (unsigned)((faces + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  if (px || py || pz)
+    hipLaunchKernelGGL(k_bc_periodic, dim3(blocks, narr), dim3(256), 0, s, b, n0, n1, n2,
+                       d.nyn, nodes ? 1 : 0, px, py, pz);
+  if (!px || !py || !pz)
+    hipLaunchKernelGGL(k_bc_faces, dim3(blocks, narr), dim3(256), 0, s, b, n0, n1, n2,
+                       zero_faces ? 1 : 0, px, py, pz);
   return hipGetLastError();
 }
 
