@@ -139,6 +139,42 @@ struct interpDensSpecies {
   FPinterp *pzz_flat;
 };
 
+// include/InterpDensNet.h:11-37: net densities (sum over species).
+struct interpDensNet {
+  FPinterp ***rhon;
+  FPinterp *rhon_flat;
+  FPinterp ***rhoc;
+  FPinterp *rhoc_flat;
+  FPinterp ***Jx;
+  FPinterp *Jx_flat;
+  FPinterp ***Jy;
+  FPinterp *Jy_flat;
+  FPinterp ***Jz;
+  FPinterp *Jz_flat;
+  FPinterp ***pxx;
+  FPinterp *pxx_flat;
+  FPinterp ***pxy;
+  FPinterp *pxy_flat;
+  FPinterp ***pxz;
+  FPinterp *pxz_flat;
+  FPinterp ***pyy;
+  FPinterp *pyy_flat;
+  FPinterp ***pyz;
+  FPinterp *pyz_flat;
+  FPinterp ***pzz;
+  FPinterp *pzz_flat;
+};
+
+// include/InterpDens_aux.h:6-13: the hat densities.  Only chained views
+// exist; the library reads and writes the contiguous block behind them, as
+// newArr3 (include/Alloc.h:52-60) allocates it.
+struct interpDens_aux {
+  FPinterp ***rhoh;
+  FPinterp ***Jxh;
+  FPinterp ***Jyh;
+  FPinterp ***Jzh;
+};
+
 struct parameters {
   double c;
   double fourpi;
@@ -250,6 +286,18 @@ int mover_interp(struct particles *part, struct EMfield *field,
                  struct parameters *param);
 void particle_sort(struct parameters *params, struct particles *particles,
                    struct grid *grd);
+
+// The field-solve consumers of the moments (include/MaxwellSolver.h:12,
+// 37): computed on the device in FP64 and written back into the host
+// arrays.  With the FP64 build the results are bit-identical to the
+// reference's; with the float build the inputs are widened, the arithmetic
+// is FP64, and the results are rounded to float once.
+void calculateHatDensities(struct interpDens_aux *id_aux, struct interpDensNet *idn,
+                           struct interpDensSpecies *ids, struct EMfield *field,
+                           struct grid *grd, struct parameters *param);
+void MaxwellImage(FPfield *im, FPfield *vector, struct EMfield *field,
+                  struct interpDensSpecies *ids, struct grid *grd,
+                  struct parameters *param);
 
 // Extensions with no counterpart in the reference.
 extern "C" {

@@ -1,5 +1,6 @@
 // spic_dropin.cpp -- the reference's C++ entry points (mover_PC, interpP2G,
-// mover_interp, particle_sort) over the C ABI of libspic.so, for host
+// mover_interp, particle_sort, and the field-solve side's
+// calculateHatDensities and MaxwellImage) over the C ABI of libspic.so, for host
 // structs laid out as the reference's (include/sputnipic/sputnipic.h).
 // Built twice by the Makefile: libspic_dropin_f32.so (the reference's
 // default float precision) and libspic_dropin_f64.so (-DSPIC_REF_FP64).
@@ -356,3 +357,174 @@ int spic_dropin_release_all(void) {
 }
 
 }  // extern "C"
+
+// ---- field-solve side ------------------------------------------------------
+
+namespace {
+
+// FP64 device buffers of the field-solve calls, grown on demand.
+struct SolveBuffers {
+  std::vector<double *> arr;  // named slots below
+  size_t nn = 0, nc = 0, nvec = 0;
+  int ns = 0;
+  void *work = nullptr;
+  size_t work_bytes = 0;
+};
+SolveBuffers g_solve;
+
+// Slots: 0..2 B, 3 rhoc, 4 rhoh, 5..7 J hat, 8 vector, 9 image,
+// 10 + 9 s .. 18 + 9 s: species s' Jx Jy Jz pxx pxy pxz pyy pyz pzz, then
+// 10 + 9 ns + s: species s' rhon.
+int solve_buffers(const grid *grd, int ns) {
+  const size_t nn = nodes(grd), nc = (size_t)grd->nxc * grd->nyc * grd->nzc;
+  const size_t nvec = 3 * (size_t)(grd->nxn - 2) * (grd->nyn - 2) * (grd->nzn - 2);
+  SolveBuffers &b = g_solve;
+  if (b.nn >= nn && b.nc >= nc && b.nvec >= nvec && b.ns >= ns) return SPIC_OK;
+  for (double *p : b.arr)
+    if (p) spic_free(p);
+  b.arr.assign(10 + 10 * (size_t)ns, nullptr);
+  int rc;
+  for (size_t i = 0; i < b.arr.size(); ++i) {
+    size_t n = nn;
+    if (i == 3 || i == 4) n = nc;
+    if (i == 8 || i == 9) n = nvec;
+    if ((rc = spic_malloc((void **)&b.arr[i], sizeof(double) * n))) return rc;
+  }
+  if (b.work) spic_free(b.work);
+  spic_grid g;
+  std::memset(&g, 0, sizeof(g));
+  g.nxc = grd->nxc; g.nyc = grd->nyc; g.nzc = grd->nzc;
+  g.nxn = grd->nxn; g.nyn = grd->nyn; g.nzn = grd->nzn;
+  b.work_bytes = spic_field_workspace_bytes(&g, ns);
+  if ((rc = spic_malloc(&b.work, b.work_bytes))) return rc;
+  b.nn = nn; b.nc = nc; b.nvec = nvec; b.ns = ns;
+  return SPIC_OK;
+}
+
+template <class T>
+int up64(double *dst, const T *src, size_t n) {
+  std::vector<double> buf(src, src + n);
+  return spic_memcpy_h2d(dst, buf.data(), sizeof(double) * n);
+}
+
+template <class T>
+int down64(T *dst, const double *src, size_t n) {
+  std::vector<double> buf(n);
+  int rc = spic_memcpy_d2h(buf.data(), src, sizeof(double) * n);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; ++i) dst[i] = (T)buf[i];
+  return SPIC_OK;
+}
+
+// The FP64 grid and parameters of the field-solve calls.
+int solve_setup(const grid *grd, const parameters *param, spic_grid *g,
+                spic_solve_params *sp) {
+  GridMirror *gm;
+  int rc = grid_mirror(grd, &gm);
+  if (rc) return rc;
+  *g = gm->g;
+  g->prec = SPIC_F64;  // the operators read no node coordinates
+  std::memset(sp, 0, sizeof(*sp));
+  sp->dt = param->dt;
+  sp->c = param->c;
+  sp->th = param->th;
+  sp->fourpi = param->fourpi;
+  sp->PERIODICX = param->PERIODICX;
+  sp->PERIODICY = param->PERIODICY;
+  sp->PERIODICZ = param->PERIODICZ;
+  if (param->ns < 1 || param->ns > 8) return SPIC_EINVAL;
+  for (int s = 0; s < param->ns; ++s) sp->qom[s] = param->qom[s];
+  return SPIC_OK;
+}
+
+int upload_b(const EMfield *field, size_t nn, spic_field *f) {
+  double **a = g_solve.arr.data();
+  int rc;
+  if ((rc = up64(a[0], field->Bxn_flat, nn)) || (rc = up64(a[1], field->Byn_flat, nn)) ||
+      (rc = up64(a[2], field->Bzn_flat, nn)))
+    return rc;
+  // the field-solve operators read B only; E slots point at B
+  f->Ex = a[0]; f->Ey = a[1]; f->Ez = a[2];
+  f->Bxn = a[0]; f->Byn = a[1]; f->Bzn = a[2];
+  return SPIC_OK;
+}
+
+}  // namespace
+
+void calculateHatDensities(struct interpDens_aux *id_aux, struct interpDensNet *idn,
+                           struct interpDensSpecies *ids, struct EMfield *field,
+                           struct grid *grd, struct parameters *param) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  spic_grid g;
+  spic_solve_params sp;
+  int rc = solve_setup(grd, param, &g, &sp);
+  const int ns = param->ns;
+  if (rc == SPIC_OK) rc = solve_buffers(grd, ns);
+  if (rc) {
+    report("calculateHatDensities", rc);
+    return;
+  }
+  double **a = g_solve.arr.data();
+  const size_t nn = nodes(grd), nc = (size_t)grd->nxc * grd->nyc * grd->nzc;
+  spic_field f;
+  std::vector<spic_moments> mom(ns);
+  rc = upload_b(field, nn, &f);
+  if (rc == SPIC_OK) rc = up64(a[3], idn->rhoc_flat, nc);
+  for (int s = 0; s < ns && rc == SPIC_OK; ++s) {
+    const FPinterp *src[9] = {ids[s].Jx_flat,  ids[s].Jy_flat,  ids[s].Jz_flat,
+                              ids[s].pxx_flat, ids[s].pxy_flat, ids[s].pxz_flat,
+                              ids[s].pyy_flat, ids[s].pyz_flat, ids[s].pzz_flat};
+    for (int m = 0; m < 9 && rc == SPIC_OK; ++m) rc = up64(a[10 + 9 * s + m], src[m], nn);
+    double *const *d = &a[10 + 9 * s];
+    mom[s].rhon = d[0];  // unread by the hat densities
+    mom[s].Jx = d[0]; mom[s].Jy = d[1]; mom[s].Jz = d[2];
+    mom[s].pxx = d[3]; mom[s].pxy = d[4]; mom[s].pxz = d[5];
+    mom[s].pyy = d[6]; mom[s].pyz = d[7]; mom[s].pzz = d[8];
+  }
+  spic_hat hat;
+  hat.rhoh = a[4]; hat.Jxh = a[5]; hat.Jyh = a[6]; hat.Jzh = a[7];
+  if (rc == SPIC_OK)
+    rc = spic_hat_densities(&g, &f, mom.data(), ns, a[3], &sp, &hat, g_solve.work,
+                            g_solve.work_bytes, stream());
+  if (rc == SPIC_OK) rc = spic_stream_synchronize(stream());
+  if (rc == SPIC_OK) rc = down64(&id_aux->rhoh[0][0][0], a[4], nc);
+  if (rc == SPIC_OK) rc = down64(&id_aux->Jxh[0][0][0], a[5], nn);
+  if (rc == SPIC_OK) rc = down64(&id_aux->Jyh[0][0][0], a[6], nn);
+  if (rc == SPIC_OK) rc = down64(&id_aux->Jzh[0][0][0], a[7], nn);
+  report("calculateHatDensities", rc);
+}
+
+void MaxwellImage(FPfield *im, FPfield *vector, struct EMfield *field,
+                  struct interpDensSpecies *ids, struct grid *grd,
+                  struct parameters *param) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  spic_grid g;
+  spic_solve_params sp;
+  int rc = solve_setup(grd, param, &g, &sp);
+  const int ns = param->ns;
+  if (rc == SPIC_OK) rc = solve_buffers(grd, ns);
+  if (rc) {
+    report("MaxwellImage", rc);
+    return;
+  }
+  double **a = g_solve.arr.data();
+  const size_t nn = nodes(grd);
+  const size_t nvec = 3 * (size_t)(grd->nxn - 2) * (grd->nyn - 2) * (grd->nzn - 2);
+  spic_field f;
+  std::vector<spic_moments> mom(ns);
+  rc = upload_b(field, nn, &f);
+  if (rc == SPIC_OK) rc = up64(a[8], vector, nvec);
+  for (int s = 0; s < ns && rc == SPIC_OK; ++s) {
+    double *r = a[10 + 9 * ns + s];
+    rc = up64(r, ids[s].rhon_flat, nn);
+    mom[s].rhon = r;
+    mom[s].Jx = mom[s].Jy = mom[s].Jz = r;  // unread by MaxwellImage
+    mom[s].pxx = mom[s].pxy = mom[s].pxz = mom[s].pyy = mom[s].pyz = mom[s].pzz = r;
+  }
+  if (rc == SPIC_OK)
+    rc = spic_maxwell_image(&g, &f, mom.data(), ns, &sp, a[8], a[9], g_solve.work,
+                            g_solve.work_bytes, stream());
+  if (rc == SPIC_OK) rc = spic_stream_synchronize(stream());
+  if (rc == SPIC_OK) rc = down64(im, a[9], nvec);
+  report("MaxwellImage", rc);
+}

@@ -1,10 +1,11 @@
 """The C++ drop-in (include/sputnipic/sputnipic.h, libspic_dropin_*.so),
 checked without a GPU:
 
-* each library exports the reference's four entry points under their C++
+* each library exports the reference's four particle entry points and the
+  field-solve side's calculateHatDensities and MaxwellImage under their C++
   mangled names, so a reference driver links against it in place of
-  src/Mover.cpp and src/Particles.cpp;
-* the five structs have the reference's layout (offset and size of every
+  src/Mover.cpp, src/Particles.cpp and those two routines;
+* the seven structs have the reference's layout (offset and size of every
   member), compared with a program compiled against the reference's own
   headers where that checkout exists.
 """
@@ -28,13 +29,23 @@ MANGLED = {
 }
 
 
+# the field-solve entry points (include/MaxwellSolver.h:12, 37); MaxwellImage
+# takes FPfield pointers, so its name depends on the precision
+MANGLED_FIELD = {
+    "f32": {"calculateHatDensities": "_Z21calculateHatDensitiesP14interpDens_auxP13interpDensNetP17interpDensSpeciesP7EMfieldP4gridP10parameters",
+            "MaxwellImage": "_Z12MaxwellImagePfS_P7EMfieldP17interpDensSpeciesP4gridP10parameters"},
+    "f64": {"calculateHatDensities": "_Z21calculateHatDensitiesP14interpDens_auxP13interpDensNetP17interpDensSpeciesP7EMfieldP4gridP10parameters",
+            "MaxwellImage": "_Z12MaxwellImagePdS_P7EMfieldP17interpDensSpeciesP4gridP10parameters"},
+}
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_dropin_exports_reference_symbols(prec):
     path = os.path.join(LIB, f"libspic_dropin_{prec}.so")
     out = subprocess.run(["nm", "-D", "--defined-only", path], check=True,
                          capture_output=True, text=True).stdout
     syms = {line.split()[-1] for line in out.splitlines() if line.strip()}
-    for name, mangled in MANGLED.items():
+    for name, mangled in list(MANGLED.items()) + list(MANGLED_FIELD[prec].items()):
         assert mangled in syms, name
     for ext in ("spic_dropin_set_resident", "spic_dropin_sync_to_host",
                 "spic_dropin_host_modified", "spic_dropin_last_status",
@@ -56,6 +67,11 @@ FIELDS = {
                           "Jx_flat", "Jy", "Jy_flat", "Jz", "Jz_flat", "pxx", "pxx_flat",
                           "pxy", "pxy_flat", "pxz", "pxz_flat", "pyy", "pyy_flat", "pyz",
                           "pyz_flat", "pzz", "pzz_flat"],
+    "interpDensNet": ["rhon", "rhon_flat", "rhoc", "rhoc_flat", "Jx", "Jx_flat", "Jy",
+                      "Jy_flat", "Jz", "Jz_flat", "pxx", "pxx_flat", "pxy", "pxy_flat",
+                      "pxz", "pxz_flat", "pyy", "pyy_flat", "pyz", "pyz_flat", "pzz",
+                      "pzz_flat"],
+    "interpDens_aux": ["rhoh", "Jxh", "Jyh", "Jzh"],
     "parameters": ["c", "fourpi", "dt", "th", "ncycles", "NiterMover", "n_sub_cycles",
                    "sort", "sort_every_n", "sort_cps", "number_of_batches",
                    "threads_per_block", "track_particles", "n_tracked_particles",
@@ -106,5 +122,6 @@ def test_struct_layout_matches_reference_headers(prec):
     if prec == "f64":
         ref_flags += ["-include", os.path.join(ROOT, "oracle", "fp64_precision.h")]
     ref = layout(['#include "Particles.h"', '#include "EMfield.h"', '#include "Grid.h"',
-                  '#include "InterpDensSpecies.h"', '#include "Parameters.h"'], ref_flags)
+                  '#include "InterpDensSpecies.h"', '#include "Parameters.h"',
+                  '#include "InterpDensNet.h"', '#include "InterpDens_aux.h"'], ref_flags)
     assert ours == ref

@@ -219,3 +219,30 @@ def test_batch_rejects_bad_members():
     batch.close()
     for x in (a, b, c, d):
         x.eng.close()
+
+
+def test_autotuner_times_every_candidate_and_picks_the_fastest():
+    """tune_batch builds the instances once per tile shape, times every
+    sort schedule, returns the fastest, and leaves the caller's state
+    alone (it advances only its own instances)."""
+    from sputnipic_amd.tune import tune_batch
+    states = [_oracle(spec, "f32") for spec in INSTANCES]
+    built = []
+
+    def make(tile):
+        insts = [Instance(st, "f32", tile) for st in states]
+        for x in insts:
+            x.close = x.eng.close
+        built.append(tile)
+        return insts
+
+    res = tune_batch(make, tiles=((4, 4, 4), (8, 4, 4)), sort_choices=(0, 2), phases=2,
+                     warmup=1, set_moments=True)
+    assert built == [(4, 4, 4), (8, 4, 4)]
+    assert len(res.trials) == 4
+    best = min(res.trials, key=lambda t: t.ms_per_phase)
+    assert (res.tile, res.sort_every, res.ms_per_phase) == (best.tile, best.sort_every,
+                                                            best.ms_per_phase)
+    assert all(t.ms_per_phase > 0 for t in res.trials)
+    d = res.as_dict()
+    assert d["tile"] in ("4x4x4", "8x4x4") and len(d["trials"]) == 4

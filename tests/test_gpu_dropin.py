@@ -102,3 +102,57 @@ def test_dropin_fused_and_sort(prec):
             assert_bitwise(parts[s][k], st.parts[s][k], f"s{s} {k}")
         for n in O.MOMENT_NAMES:
             assert_close(moms[s][n], st.moments[s][n], tol, f"s{s} {n}")
+
+
+def run_field_driver(st: O.State, prec: str, rhoc, vec):
+    c = st.cfg
+    dt = np.float64 if prec == "f64" else np.float32
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("7i", c.nxc, c.nyc, c.nzc, c.ns, int(c.PERIODICX),
+                                int(c.PERIODICY), int(c.PERIODICZ)))
+            f.write(struct.pack(f"{7 + c.ns}d", c.Lx, c.Ly, c.Lz, c.dt, c.c, c.th, c.fourpi,
+                                *c.qom[:c.ns]))
+            for n in ("Bxn", "Byn", "Bzn"):
+                f.write(np.ascontiguousarray(st.fields[n], dt).tobytes())
+            for s in range(c.ns):
+                for m in O.MOMENT_NAMES:
+                    f.write(np.ascontiguousarray(st.moments[s][m], dt).tobytes())
+            f.write(np.ascontiguousarray(rhoc, dt).tobytes())
+            f.write(np.ascontiguousarray(vec, dt).tobytes())
+        exe = os.path.join(LIB, f"spic_dropin_field_driver_{prec}")
+        r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        raw = np.fromfile(fout, dtype=dt)
+    nn, nc = st.nn, st.ncell
+    out, off = {}, 0
+    for name, n in (("rhoh", nc), ("Jxh", nn), ("Jyh", nn), ("Jzh", nn), ("im", len(vec))):
+        out[name] = raw[off:off + n]
+        off += n
+    assert off == raw.size
+    return out
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dropin_field_solve(prec):
+    """calculateHatDensities and MaxwellImage through the C++ entry points,
+    on the moments of one cycle.  FP64: bit-identical to the oracle (and so
+    to the reference).  Float build: the library widens the inputs, works in
+    FP64 and rounds once, while the float reference rounds every operation:
+    within 1e-5 of each array's largest magnitude."""
+    st = small_state(prec)
+    st.cycle()
+    net = st.sum_species()
+    _, rhoc = st.rhoc_from_rhon(net["rhon"])
+    g = st.grid
+    nvec = 3 * (g.nxn - 2) * (g.nyn - 2) * (g.nzn - 2)
+    vec = np.random.default_rng(4).normal(0, 1e-2, nvec).astype(st.dt)
+    got = run_field_driver(st, prec, rhoc, vec)
+    want = st.hat_densities(rhoc)
+    want["im"] = st.maxwell_image(vec)
+    for n in ("rhoh", "Jxh", "Jyh", "Jzh", "im"):
+        if prec == "f64":
+            assert_bitwise(got[n], want[n], n)
+        else:
+            assert_close(got[n], want[n], 1e-5, n)
