@@ -487,6 +487,100 @@ def run_batch(steps: int, warmup: int) -> dict:
     }
 
 
+def run_order(prec: str, phases: int = 3) -> dict:
+    """Input order: the same particles generated in initial-condition order
+    (cell-major, as the reference's initGEM lays them out) and randomly
+    permuted on the device (an affine bijection of the indices).  For each:
+    the engine's load (copy in and bin by tile) and its first phases, and
+    the plain caller-order fused kernel (mover_interp, global atomics) on
+    species 0.  The engine's binning makes its phases independent of the
+    input order; the caller-order kernel is not."""
+    import ctypes as C
+
+    import sputnipic_amd as S
+    from sputnipic_amd._capi import check, lib
+
+    L = lib()
+    grd = S.Grid(*GRID, *BOX, prec)
+    gem = S.GemParams()
+    field = S.gem_fields(grd, gem)
+    param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
+    stream = S.Stream()
+    out = {}
+    g_c = grd.c_struct()
+    for order in ("ic_order", "permuted"):
+        parts = []
+        for s_, sp in enumerate(S.harris_species(PPC)):
+            p = S.gem_particles(grd, gem, sp, s_, seed=0)
+            if order == "permuted":
+                q = S.DeviceParticles(p.npmax, p.qom, prec)
+                a, b = 2654435761 % p.npmax, 12345
+                while np_gcd(a, p.npmax) != 1:
+                    a += 1
+                check(L.spic_permute_species(C.byref(g_c), C.byref(p.c_struct()),
+                                             C.byref(q.c_struct()), a, b, None), "permute")
+                for d in p.d.values():
+                    d.free()
+                p = q
+            parts.append(p)
+        # caller-order fused kernel on species 0
+        moms0 = S.InterpDensSpecies(grd)
+        ev = [S.Event(), S.Event()]
+        S.set_async(True)
+        try:
+            ev[0].record(stream)
+            check(L.spic_mover_interp(C.byref(g_c), C.byref(field.c_struct()),
+                                      C.byref(parts[0].c_struct()), C.byref(moms0.c_struct()),
+                                      C.byref(param.push()), stream.ptr), "mover_interp")
+            ev[1].record(stream)
+            stream.synchronize()
+            check(L.spic_check_domain(stream.ptr), "domain")
+        finally:
+            S.set_async(False)
+        plain_ms = ev[0].elapsed_ms(ev[1])
+        for d in moms0.d.values():
+            d.free()
+        eng = S.Engine(grd, NS, set_moments=True)
+        t0 = time.perf_counter()
+        for s_, p in enumerate(parts):
+            eng.load(s_, p, stream)
+        stream.synchronize()
+        t_load = time.perf_counter() - t0
+        for p in parts:
+            for d in p.d.values():
+                d.free()
+        moms = [S.InterpDensSpecies(grd) for _ in range(NS)]
+        ms = []
+        S.set_async(True)
+        try:
+            for _ in range(phases):
+                a_, b_ = S.Event(), S.Event()
+                a_.record(stream)
+                eng.phase(field, moms, param, stream)
+                b_.record(stream)
+                stream.synchronize()
+                ms.append(a_.elapsed_ms(b_))
+            S.check_domain(stream)
+        finally:
+            S.set_async(False)
+        eng.close()
+        for m in moms:
+            for d in m.d.values():
+                d.free()
+        out[order] = {"engine_load_and_bin_s": t_load, "engine_phase_ms": ms,
+                      "plain_mover_interp_species0_ms": plain_ms}
+        log(f"{prec} {order}: engine load+bin {t_load:.3f} s, phases "
+            + ", ".join(f"{v:.2f}" for v in ms) + f" ms; caller-order mover_interp of "
+            f"species 0 {plain_ms:.1f} ms")
+    return out
+
+
+def np_gcd(a: int, b: int) -> int:
+    while b:
+        a, b = b, a % b
+    return a
+
+
 def field_bytes(grd, ns: int):
     """Algorithmic bytes of the field-solve operators on grd (FP64): the
     net rhoc step + hat densities read the net rhon, every species' J and
@@ -700,6 +794,8 @@ def main() -> int:
                          "batch: 16 independent Harris-sheet instances in FP32, one "
                          "launch per kernel and phase, autotuned, in one HIP graph; "
                          "field: only the field-solve operators on the 128x64x64 grid")
+    ap.add_argument("--no-order", action="store_true",
+                    help="skip the IC-order against permuted-input comparison")
     ap.add_argument("--no-field", action="store_true",
                     help="skip the field-solve operators (hat densities, MaxwellImage)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -845,6 +941,10 @@ def main() -> int:
     if not args.no_cpu_baseline:
         with _StdoutToStderr():
             cpu = cpu_baseline(args.cpu_cycles)
+    order = None
+    if not args.no_order and args.engine == "tiled":
+        with _StdoutToStderr():
+            order = run_order("f32" if "f32" in forms else primary)
     field = None
     if not args.no_field:
         with _StdoutToStderr():
@@ -885,6 +985,7 @@ def main() -> int:
         "forms": {k: {kk: vv for kk, vv in v.items()} for k, v in forms.items()},
         "cpu_baseline": cpu,
         "field_solve": field,
+        "input_order": order,
     }
     if "f64" in forms and "f32" in forms:
         line["fp32_over_fp64"] = forms["f32"]["value"] / forms["f64"]["value"]

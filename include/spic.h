@@ -147,6 +147,13 @@ int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
 int spic_snapshot_be32(const spic_grid *grd, const spic_species *sp,
                        void *const dst[6], void *stream);
 
+/* dst[(a i + b) mod nop] = src[i] for x..w and q: a bijection when a is
+ * coprime to nop.  A utility for measuring on input in random order (the
+ * engine's sort on load removes any input order). */
+int spic_permute_species(const spic_grid *grd, const spic_species *src,
+                         const spic_species *dst, uint64_t a, uint64_t b,
+                         void *stream);
+
 /* setZeroSpeciesDensities, node-moment part: zero the ten arrays. */
 int spic_zero_moments(const spic_grid *grd, spic_moments *mom, void *stream);
 

@@ -150,6 +150,8 @@ SIGNATURES = {
     "spic_engine_snapshot_be32": (C.c_int, [_P, C.c_int, C.POINTER(_P), _P]),
     "spic_snapshot_be32": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_species),
                                      C.POINTER(_P), _P]),
+    "spic_permute_species": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_species),
+                                       C.POINTER(spic_species), C.c_uint64, C.c_uint64, _P]),
     "spic_host_alloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
     "spic_host_free": (C.c_int, [_P]),
     "spic_memcpy_d2h_async": (C.c_int, [_P, _P, C.c_size_t, _P]),

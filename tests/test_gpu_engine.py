@@ -239,3 +239,41 @@ def test_engine_reports_out_of_domain():
     with pytest.raises(S.SpicError) as ei:
         run_engine(st, "f64", phases=1)
     assert ei.value.code == S.SPIC_EDOMAIN
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_input_order_does_not_matter(prec):
+    """Particles in random order (spic_permute_species: index i to
+    (a i + b) mod n) give the engine the same trajectories, particle for
+    particle, as the oracle run on the same permuted arrays."""
+    import ctypes as C
+    from sputnipic_amd._capi import check, lib
+
+    st = oracle_small(prec)
+    c = st.cfg
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
+    g_c = grd.c_struct()
+    a, b = 7919, 101
+    for s in range(c.ns):
+        n = len(st.parts[s]["x"])
+        src = S.Particles(st.parts[s], c.qom[s], prec, c.NiterMover)
+        dst = S.DeviceParticles(n, c.qom[s], prec)
+        check(lib().spic_permute_species(C.byref(g_c), C.byref(src.c_struct()),
+                                         C.byref(dst.c_struct()), a, b, None), "permute")
+        got = dst.download()
+        j = (a * np.arange(n, dtype=np.uint64) + b) % np.uint64(n)
+        assert np.unique(j).size == n
+        for k in O.PARTICLE_NAMES:
+            want = np.empty_like(st.parts[s][k])
+            want[j] = st.parts[s][k]
+            assert_bitwise(got[k], want, f"permuted s{s} {k}")
+            st.parts[s][k] = want
+    S.set_strict_fp(True)
+    got_p, got_m, _ = run_engine(st, prec, phases=2)
+    oracle_phases(st, 2)
+    for s in range(c.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
