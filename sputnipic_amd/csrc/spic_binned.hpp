@@ -80,6 +80,12 @@ __host__ __device__ inline RangeView range_view(PS seg, PC cnt, int ntiles,
   return v;
 }
 
+// Storage type of the tile partials: the particle precision.  An FP32
+// engine sums its LDS block in FP64 by default and rounds each tile's block
+// to float once; the node reduction adds the tiles' floats in FP64.
+template <typename T>
+using PartialT = T;
+
 template <typename T>
 struct BinnedSpecies {
   // x y z u v w q, binned: component c of slot i at in[c * stride + i]
@@ -114,16 +120,18 @@ struct BinnedArgs {
   // deposits with global atomics, [2] stencil gathers from LDS, [3]
   // deposits through LDS (bucket or LDS atomics)
   unsigned long long *counters;
-  // per-tile moment partials [tile][species][moment][block node], FP64,
-  // written for every tile-species whose input segment is not empty
-  double *partials;
+  // per-tile moment partials [tile][species][moment][block node] in the
+  // particle precision (PartialT<T>), written for every tile-species whose
+  // input segment is not empty
+  void *partials;
 };
 
 struct BinnedReduceArgs {
   int nyn, nzn;
   int ntx, nty, ntz, ntiles;
   int ns;
-  const double *partials;
+  int partials_fp32;  // partials stored as float (FP32 engines), else double
+  const void *partials;
   const int64_t *seg[kMaxSpecies];
   const unsigned int *cnt[kMaxSpecies];
   KMom mom[kMaxSpecies];

@@ -663,8 +663,9 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           atomicAdd(&s_mom[m * NB + nb], (A)acc[cc * 10 + m] * scale);
     }
     __syncthreads();
-    SPIC_GLOBAL double *dst = gmem(a.partials) + ((size_t)tile * a.ns + s) * (10 * NB);
-    for (int i = tid; i < 10 * NB; i += NT) dst[i] = (double)s_mom[i];
+    using P = PartialT<T>;
+    SPIC_GLOBAL P *dst = gmem((P *)a.partials) + ((size_t)tile * a.ns + s) * (10 * NB);
+    for (int i = tid; i < 10 * NB; i += NT) dst[i] = (P)s_mom[i];
     if (tid == 0) {
       // the reserved block [stay_base, stay_base + nin) holds s_stay
       // particles; its remainder inside the segment stays unused

@@ -208,10 +208,10 @@ __global__ void k_zero_counts(CountArrays c) {
 // Neumaier's compensated summation, added to the FP64 moment arrays.
 // Tiles whose input segment held no particle of the species wrote no
 // partials and are skipped.
-template <int TX, int TY, int TZ>
+template <typename P, int TX, int TY, int TZ>
 __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int ntx, int nty,
                                             int ntz, int ntiles, int ns,
-                                            const double *__restrict__ partials,
+                                            const P *__restrict__ partials,
                                             const int64_t *seg, const unsigned int *cnt,
                                             double *const *mom) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
@@ -233,10 +233,10 @@ __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int 
         if (range_view(gmem(seg), gmem(cnt), ntiles, 0u, tile).count == 0) continue;
         any = true;
         const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
-        const SPIC_GLOBAL double *src = gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + b;
+        const SPIC_GLOBAL P *src = gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + b;
 #pragma unroll
         for (int m = 0; m < 10; ++m) {
-          const double v = src[m * NB];
+          const double v = (double)src[m * NB];
           const double t = sum[m] + v;
           cmp[m] += fabs(sum[m]) >= fabs(v) ? (sum[m] - t) + v : (v - t) + sum[m];
           sum[m] = t;
@@ -247,13 +247,13 @@ __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int 
   for (int m = 0; m < 10; ++m) gmem(mom[m])[n] += sum[m] + cmp[m];
 }
 
-template <int TX, int TY, int TZ>
+template <typename P, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int nn) {
   const int s = blockIdx.y;
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= nn) return;
-  reduce_node<TX, TY, TZ>(n, s, a.nyn, a.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
-                          a.partials, a.seg[s], a.cnt[s], a.mom[s].m);
+  reduce_node<P, TX, TY, TZ>(n, s, a.nyn, a.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
+                             (const P *)a.partials, a.seg[s], a.cnt[s], a.mom[s].m);
 }
 
 // The reduce of a batch: blockIdx.y is the species.
@@ -266,8 +266,9 @@ __global__ __launch_bounds__(256) void k_binned_reduce_batch(
   const int s = blockIdx.y;
   const int n = (blockIdx.x - map.start[i]) * 256 + threadIdx.x;
   if (s >= a.ns || n >= a.g.nxn * a.g.nyn * a.g.nzn) return;
-  reduce_node<TX, TY, TZ>(n, s, a.g.nyn, a.g.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
-                          a.partials, a.sp[s].seg, a.sp[s].cnt_in, a.sp[s].m.m);
+  reduce_node<PartialT<T>, TX, TY, TZ>(n, s, a.g.nyn, a.g.nzn, a.ntx, a.nty, a.ntz, a.ntiles,
+                                       a.ns, (const PartialT<T> *)a.partials, a.sp[s].seg,
+                                       a.sp[s].cnt_in, a.sp[s].m.m);
 }
 
 // The count reset of a batch (blockIdx.y is the species), with the moment
@@ -297,12 +298,16 @@ hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int t
                                 int nn, hipStream_t s) {
   if (nn <= 0 || a.ns <= 0) return hipSuccess;
   dim3 grid((nn + 255) / 256, a.ns);
-  if (tx == 4 && ty == 4 && tz == 4)
-    hipLaunchKernelGGL((k_binned_reduce<4, 4, 4>), grid, dim3(256), 0, s, a, nn);
-  else if (tx == 8 && ty == 4 && tz == 4)
-    hipLaunchKernelGGL((k_binned_reduce<8, 4, 4>), grid, dim3(256), 0, s, a, nn);
-  else
+  const bool f = a.partials_fp32 != 0;
+  if (tx == 4 && ty == 4 && tz == 4) {
+    if (f) hipLaunchKernelGGL((k_binned_reduce<float, 4, 4, 4>), grid, dim3(256), 0, s, a, nn);
+    else hipLaunchKernelGGL((k_binned_reduce<double, 4, 4, 4>), grid, dim3(256), 0, s, a, nn);
+  } else if (tx == 8 && ty == 4 && tz == 4) {
+    if (f) hipLaunchKernelGGL((k_binned_reduce<float, 8, 4, 4>), grid, dim3(256), 0, s, a, nn);
+    else hipLaunchKernelGGL((k_binned_reduce<double, 8, 4, 4>), grid, dim3(256), 0, s, a, nn);
+  } else {
     return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -45,7 +45,7 @@ struct spic_engine {
     int segi = 0;                  // layout in use
   } sp[spic::kMaxSpecies];
   unsigned long long *counters = nullptr;  // see BinnedArgs::counters
-  double *partials = nullptr;      // see BinnedArgs::partials
+  void *partials = nullptr;        // see BinnedArgs::partials
   size_t partials_bytes = 0;
   void *ws = nullptr;              // sort workspace
   size_t ws_bytes = 0;
@@ -215,6 +215,7 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   r.ntx = e->L.ntx; r.nty = e->L.nty; r.ntz = e->L.ntz;
   r.ntiles = e->L.ntiles;
   r.partials = e->partials;
+  r.partials_fp32 = e->esz == 4 ? 1 : 0;
   const int nn = e->grd.nxn * e->grd.nyn * e->grd.nzn;
   err = spic::launch_binned_reduce(r, e->L.tx, e->L.ty, e->L.tz, nn, s);
   if (err != hipSuccess) return err;
@@ -299,7 +300,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->grd.xn = e->xn;
   e->grd.yn = e->yn;
   e->grd.zn = e->zn;
-  e->partials_bytes = sizeof(double) * 10 * (size_t)ns * L.ntiles *
+  e->partials_bytes = (size_t)e->esz * 10 * (size_t)ns * L.ntiles *
                      spic::block_nodes(tx, ty, tz);
   if ((err = dmalloc(e, (void **)&e->partials, e->partials_bytes)) != hipSuccess) {
     spic_engine_destroy(e);
