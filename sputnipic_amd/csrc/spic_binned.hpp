@@ -184,10 +184,10 @@ hipError_t zero_counts(const CountArrays &c, hipStream_t s);
 
 // The reduce and the count reset of a batch, from the device-resident
 // argument blocks tab[i] of its instances: per instance, the node blocks
-// nodes.start[i].. of 256 nodes each (reduce) and the counter blocks
-// counts.start[i].. of 256 entries each (reset of every species' cnt_out,
-// 3 * (ntiles + 1) entries, then with `moments` the ten moment arrays of
-// every species); max_ns species per launch row.
+// nodes.start[i].. of 256 nodes each (reduce) and the reset blocks
+// counts.start[i].. of 256 threads each, thread j resetting counter j of
+// every species' cnt_out (3 * (ntiles + 1) entries) and, with `moments`,
+// node j of its ten moment arrays; max_ns species per launch row.
 template <typename T>
 hipError_t launch_batch_reduce(const BinnedArgs<T> *tab, const BatchMap &nodes,
                                int max_ns, int tx, int ty, int tz, hipStream_t s);

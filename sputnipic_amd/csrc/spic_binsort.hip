@@ -186,19 +186,18 @@ __global__ void k_bin_store_be32(Arrays7<T> src, const uint32_t *__restrict__ id
   }
 }
 
-// Per species (blockIdx.y): the n counters, then (c.nn > 0) the ten
-// moment arrays of nn nodes each, zeroed.
+// Per species (blockIdx.y): thread i zeroes counter i (i < n) and, with
+// c.nn > 0, node i of the ten moment arrays (i < nn).
 __global__ void k_zero_counts(CountArrays c) {
   const int s = blockIdx.y;
   if (s >= c.ns) return;
-  const int64_t total = c.n + 10 * c.nn;
+  const int64_t total = c.n > c.nn ? c.n : c.nn;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
-    if (i < c.n) {
-      gmem(c.p[s])[i] = 0u;
-    } else {
-      const int64_t j = i - c.n;
-      gmem(c.mom[s].m[j / c.nn])[j % c.nn] = 0.0;
+    if (i < c.n) gmem(c.p[s])[i] = 0u;
+    if (i < c.nn) {
+#pragma unroll
+      for (int m = 0; m < 10; ++m) gmem(c.mom[s].m[m])[i] = 0.0;
     }
   }
 }
@@ -271,9 +270,9 @@ __global__ __launch_bounds__(256) void k_binned_reduce_batch(
                                        a.sp[s].cnt_in, a.sp[s].m.m);
 }
 
-// The count reset of a batch (blockIdx.y is the species), with the moment
-// arrays after the counters when `moments` is set: element j of instance i
-// is counter j, or node (j - counters) % nn of moment (j - counters) / nn.
+// The count reset of a batch (blockIdx.y is the species): thread j of
+// instance i zeroes counter j and, when `moments` is set, node j of the ten
+// moment arrays.
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_counts_batch(
     const BinnedArgs<T> *__restrict__ tab, BatchMap map, int moments) {
@@ -283,12 +282,10 @@ __global__ __launch_bounds__(256) void k_zero_counts_batch(
   const int s = blockIdx.y;
   if (s >= a.ns) return;
   const int64_t j = (int64_t)(blockIdx.x - map.start[i]) * 256 + threadIdx.x;
-  const int64_t nc = 3 * (int64_t)(a.ntiles + 1);
-  const int64_t nn = (int64_t)a.g.nxn * a.g.nyn * a.g.nzn;
-  if (j < nc) {
-    gmem(a.sp[s].cnt_out)[j] = 0u;
-  } else if (moments && j < nc + 10 * nn) {
-    gmem(a.sp[s].m.m[(j - nc) / nn])[(j - nc) % nn] = 0.0;
+  if (j < 3 * (int64_t)(a.ntiles + 1)) gmem(a.sp[s].cnt_out)[j] = 0u;
+  if (moments && j < (int64_t)a.g.nxn * a.g.nyn * a.g.nzn) {
+#pragma unroll
+    for (int m = 0; m < 10; ++m) gmem(a.sp[s].m.m[m])[j] = 0.0;
   }
 }
 
@@ -349,8 +346,8 @@ template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
 
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n + 10 * c.nn), c.ns), dim3(256), 0, s,
-                     c);
+  hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n > c.nn ? c.n : c.nn), c.ns), dim3(256),
+                     0, s, c);
   return hipGetLastError();
 }
 

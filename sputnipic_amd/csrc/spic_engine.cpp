@@ -565,8 +565,9 @@ hipError_t build_tables(spic_batch *b, unsigned int *status) {
     b->tiles.start[i + 1] = b->tiles.start[i] + (any ? e->L.ntiles : 0);
     b->ovf.start[i + 1] = b->ovf.start[i] + (any ? kBatchOverflowBlocks : 0);
     b->nodes.start[i + 1] = b->nodes.start[i] + (any ? blocks(nn, 256) : 0);
-    const int64_t zeroed = 3 * (int64_t)(e->L.ntiles + 1) +
-                           ((b->flags & SPIC_BATCH_SET_MOMENTS) ? 10 * nn : 0);
+    const int64_t ncnt = 3 * (int64_t)(e->L.ntiles + 1);
+    const int64_t zeroed =
+        (b->flags & SPIC_BATCH_SET_MOMENTS) && nn > ncnt ? nn : ncnt;
     b->counts.start[i + 1] = b->counts.start[i] + (any ? blocks(zeroed, 256) : 0);
     b->gen[i] = e->gen;
   }
