@@ -124,6 +124,9 @@ struct BinnedArgs {
   // particle precision (PartialT<T>), written for every tile-species whose
   // input segment is not empty
   void *partials;
+  // [tile][species]: 1 when the tile wrote partials for the species this
+  // phase (its input segment was not empty), else 0
+  unsigned char *pflag;
 };
 
 struct BinnedReduceArgs {
@@ -132,8 +135,7 @@ struct BinnedReduceArgs {
   int ns;
   int partials_fp32;  // partials stored as float (FP32 engines), else double
   const void *partials;
-  const int64_t *seg[kMaxSpecies];
-  const unsigned int *cnt[kMaxSpecies];
+  const unsigned char *pflag;  // see BinnedArgs::pflag
   KMom mom[kMaxSpecies];
 };
 

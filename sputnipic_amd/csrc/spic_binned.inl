@@ -464,6 +464,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     const BinnedSpecies<T> &sp = a.sp[s];
     const RangeView in = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, tile);
     const int64_t nin = in.count;
+    if (tid == 0) gmem(a.pflag)[(size_t)tile * a.ns + s] = nin > 0 ? 1 : 0;
     if (nin == 0) continue;  // uniform across the workgroup
     for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
     for (int i = tid; i < 2 * NC; i += NT) (&s_cnt[0][0])[i] = 0u;

@@ -206,12 +206,12 @@ __global__ void k_zero_counts(CountArrays c) {
 // block holds the node (at most two tiles per axis), in tile order, with
 // Neumaier's compensated summation, added to the FP64 moment arrays.
 // Tiles whose input segment held no particle of the species wrote no
-// partials and are skipped.
+// partials (their flag is 0) and are skipped.
 template <typename P, int TX, int TY, int TZ>
 __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int ntx, int nty,
                                             int ntz, int ntiles, int ns,
                                             const P *__restrict__ partials,
-                                            const int64_t *seg, const unsigned int *cnt,
+                                            const unsigned char *__restrict__ pflag,
                                             double *const *mom) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
   const int gz = n % nzn, gy = (n / nzn) % nyn, gx = n / (nzn * nyn);
@@ -229,7 +229,7 @@ __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int 
     for (int ty = yl; ty <= yh; ++ty)
       for (int tz = zl; tz <= zh; ++tz) {
         const int tile = (tx * nty + ty) * ntz + tz;
-        if (range_view(gmem(seg), gmem(cnt), ntiles, 0u, tile).count == 0) continue;
+        if (!gmem(pflag)[(size_t)tile * ns + s]) continue;
         any = true;
         const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
         const SPIC_GLOBAL P *src = gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + b;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_binned_reduce(BinnedReduceArgs a, int n
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= nn) return;
   reduce_node<P, TX, TY, TZ>(n, s, a.nyn, a.nzn, a.ntx, a.nty, a.ntz, a.ntiles, a.ns,
-                             (const P *)a.partials, a.seg[s], a.cnt[s], a.mom[s].m);
+                             (const P *)a.partials, a.pflag, a.mom[s].m);
 }
 
 // The reduce of a batch: blockIdx.y is the species.
@@ -266,8 +266,8 @@ __global__ __launch_bounds__(256) void k_binned_reduce_batch(
   const int n = (blockIdx.x - map.start[i]) * 256 + threadIdx.x;
   if (s >= a.ns || n >= a.g.nxn * a.g.nyn * a.g.nzn) return;
   reduce_node<PartialT<T>, TX, TY, TZ>(n, s, a.g.nyn, a.g.nzn, a.ntx, a.nty, a.ntz, a.ntiles,
-                                       a.ns, (const PartialT<T> *)a.partials, a.sp[s].seg,
-                                       a.sp[s].cnt_in, a.sp[s].m.m);
+                                       a.ns, (const PartialT<T> *)a.partials, a.pflag,
+                                       a.sp[s].m.m);
 }
 
 // The count reset of a batch (blockIdx.y is the species): thread j of

@@ -47,6 +47,7 @@ struct spic_engine {
   unsigned long long *counters = nullptr;  // see BinnedArgs::counters
   void *partials = nullptr;        // see BinnedArgs::partials
   size_t partials_bytes = 0;
+  unsigned char *pflag = nullptr;  // see BinnedArgs::pflag
   void *ws = nullptr;              // sort workspace
   size_t ws_bytes = 0;
   std::mutex mu;
@@ -153,6 +154,7 @@ int fill_args(const spic_engine *e, const spic_field *field, const spic_moments 
   a.status = status;
   a.counters = e->counters;
   a.partials = e->partials;
+  a.pflag = e->pflag;
   int k = 0;
   for (int is = 0; is < e->ns; ++is) {
     const auto &sp = e->sp[is];
@@ -195,8 +197,6 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   spic::CountArrays zc;
   std::memset(&zc, 0, sizeof(zc));
   for (int j = 0; j < k; ++j) {
-    r.cnt[j] = a.sp[j].cnt_in;
-    r.seg[j] = a.sp[j].seg;
     r.mom[j] = a.sp[j].m;
     zc.p[j] = a.sp[j].cnt_out;
     zc.mom[j] = a.sp[j].m;
@@ -216,6 +216,7 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   r.ntiles = e->L.ntiles;
   r.partials = e->partials;
   r.partials_fp32 = e->esz == 4 ? 1 : 0;
+  r.pflag = e->pflag;
   const int nn = e->grd.nxn * e->grd.nyn * e->grd.nzn;
   err = spic::launch_binned_reduce(r, e->L.tx, e->L.ty, e->L.tz, nn, s);
   if (err != hipSuccess) return err;
@@ -302,7 +303,8 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->grd.zn = e->zn;
   e->partials_bytes = (size_t)e->esz * 10 * (size_t)ns * L.ntiles *
                      spic::block_nodes(tx, ty, tz);
-  if ((err = dmalloc(e, (void **)&e->partials, e->partials_bytes)) != hipSuccess) {
+  if ((err = dmalloc(e, (void **)&e->partials, e->partials_bytes)) != hipSuccess ||
+      (err = dmalloc(e, (void **)&e->pflag, (size_t)ns * L.ntiles)) != hipSuccess) {
     spic_engine_destroy(e);
     return hip_fail(err, "engine tile partials");
   }
@@ -326,6 +328,7 @@ int spic_engine_destroy(spic_engine *e) {
   if (e->ws) (void)hipFree(e->ws);
   if (e->counters) (void)hipFree(e->counters);
   if (e->partials) (void)hipFree(e->partials);
+  if (e->pflag) (void)hipFree(e->pflag);
   delete e;
   return SPIC_OK;
 }
