@@ -29,6 +29,7 @@ constexpr int kMaxSpecies = 8;
 constexpr int kBucketCap = 7;  // LDS bucket slots per cell and batch
 constexpr int kListCap = 128;  // LDS list ring entries
 constexpr int kMaxInstances = 16;  // engines in one batched launch
+constexpr int kMaxTileCells = 128;  // cells of the largest tile shape (8x4x4)
 
 // Blocks of a batched launch: instance i owns blocks [start[i], start[i+1]),
 // i < n; start[n] is the grid size.

@@ -52,44 +52,98 @@ __global__ void k_range_counts(const int64_t *__restrict__ seg,
     out[t] = (unsigned int)range_view(seg, cnt, ntiles, ovf_cap, t).count;
 }
 
-// Source slot of dense particle i: its range is the last t with P[t] <= i.
-__device__ __forceinline__ int64_t source_slot(int64_t i, const unsigned int *P,
-                                               const int64_t *seg,
-                                               const unsigned int *cnt,
-                                               unsigned int ovf_cap, int ntiles) {
-  int lo = 0, hi = ntiles;  // answer in [0, ntiles]
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if ((int64_t)P[mid] <= i) lo = mid;
-    else hi = mid - 1;
+
+// Rank of this lane's particle among the active lanes of its wave that
+// share `key`, plus the count already in counter[key]: one atomic per
+// distinct key in the wave (few when the input is in cell order).
+__device__ __forceinline__ uint32_t wave_rank(unsigned int *counter, uint32_t key,
+                                              bool active) {
+  const int lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(active);
+  uint32_t rank = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(active && key == lkey);
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(&counter[lkey], (unsigned int)__popcll(same));
+    base = __shfl(base, leader);
+    if (active && key == lkey) rank = base + (uint32_t)__popcll(same & below);
+    todo &= ~same;
   }
-  return range_view(seg, cnt, ntiles, ovf_cap, lo).slot(i - (int64_t)P[lo]);
+  return rank;
 }
 
+// Key of one particle: tile | rank of the particle within its cell | cell
+// within the tile.
+template <typename T>
+__device__ __forceinline__ void cell_of(const KGrid<T> &g, const BinLayout &L, T x, T y, T z,
+                                        int &cx, int &cy, int &cz, uint32_t &tile,
+                                        uint32_t &cit) {
+  cx = phys_cell<T>(x, g.xStart, g.invdx, L.ncx);
+  cy = phys_cell<T>(y, g.yStart, g.invdy, L.ncy);
+  cz = phys_cell<T>(z, g.zStart, g.invdz, L.ncz);
+  const int tx = cx / L.tx, ty = cy / L.ty, tz = cz / L.tz;
+  tile = (uint32_t)((tx * L.nty + ty) * L.ntz + tz);
+  cit = (uint32_t)(((cx - tx * L.tx) * L.ty + (cy - ty * L.ty)) * L.tz + (cz - tz * L.tz));
+}
+
+// Keys of a dense source (particles 0..n-1 in the caller's order), ranks
+// from per-cell counters with wave-aggregated atomics.
 template <typename T>
 __global__ void k_bin_keys(KGrid<T> g, BinLayout L, int64_t n,
                            const T *__restrict__ x, const T *__restrict__ y,
-                           const T *__restrict__ z,
-                           const unsigned int *__restrict__ P,
-                           const int64_t *__restrict__ seg,
-                           const unsigned int *__restrict__ cnt, unsigned int ovf_cap,
-                           unsigned int *__restrict__ cell_count,
+                           const T *__restrict__ z, unsigned int *__restrict__ cell_count,
                            uint32_t *__restrict__ key, uint32_t *__restrict__ idx) {
   const uint32_t rank_max = (1u << L.rank_bits) - 1u;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = P ? source_slot(i, P, seg, cnt, ovf_cap, L.ntiles) : i;
-    const int cx = phys_cell<T>(x[j], g.xStart, g.invdx, L.ncx);
-    const int cy = phys_cell<T>(y[j], g.yStart, g.invdy, L.ncy);
-    const int cz = phys_cell<T>(z[j], g.zStart, g.invdz, L.ncz);
-    const int tx = cx / L.tx, ty = cy / L.ty, tz = cz / L.tz;
-    const uint32_t tile = (uint32_t)((tx * L.nty + ty) * L.ntz + tz);
-    const uint32_t cit = (uint32_t)(((cx - tx * L.tx) * L.ty + (cy - ty * L.ty)) * L.tz +
-                                    (cz - tz * L.tz));
-    uint32_t r = atomicAdd(&cell_count[((size_t)cx * L.ncy + cy) * L.ncz + cz], 1u);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n_up = (n + 63) / 64 * 64;  // whole waves stay in the loop
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_up; i += stride) {
+    const bool active = i < n;
+    int cx = 0, cy = 0, cz = 0;
+    uint32_t tile = 0, cit = 0;
+    if (active) cell_of<T>(g, L, x[i], y[i], z[i], cx, cy, cz, tile, cit);
+    const uint32_t cell = (uint32_t)(((int64_t)cx * L.ncy + cy) * L.ncz + cz);
+    uint32_t r = wave_rank(cell_count, cell, active);
+    if (!active) continue;
     r = r < rank_max ? r : rank_max;
     key[i] = (tile << (L.rank_bits + L.cell_bits)) | (r << L.cell_bits) | cit;
-    idx[i] = (uint32_t)j;
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// Keys of a binned source: block t takes range t (a tile's segment, or
+// t == ntiles: the overflow region), whose particles land at dense
+// positions P[t] + r.  A tile's own particles take their rank from LDS
+// counters of its cells; others (overflow region) from the global ones.
+// Equal ranks of one cell from the two sources only make equal keys.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_keys_binned(
+    KGrid<T> g, BinLayout L, const T *__restrict__ x, const T *__restrict__ y,
+    const T *__restrict__ z, const unsigned int *__restrict__ P,
+    const int64_t *__restrict__ seg, const unsigned int *__restrict__ cnt,
+    unsigned int ovf_cap, unsigned int *__restrict__ cell_count, uint32_t *__restrict__ key,
+    uint32_t *__restrict__ idx) {
+  __shared__ unsigned int s_cnt[kMaxTileCells];
+  const uint32_t rank_max = (1u << L.rank_bits) - 1u;
+  const int t = blockIdx.x;
+  const int ncells = L.tx * L.ty * L.tz;
+  for (int c = threadIdx.x; c < ncells; c += blockDim.x) s_cnt[c] = 0u;
+  __syncthreads();
+  const RangeView v = range_view(seg, cnt, L.ntiles, ovf_cap, t);
+  const int64_t base = P[t];
+  for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
+    const int64_t j = v.slot(r);
+    int cx, cy, cz;
+    uint32_t tile, cit;
+    cell_of<T>(g, L, x[j], y[j], z[j], cx, cy, cz, tile, cit);
+    uint32_t rk;
+    if ((int)tile == t) rk = atomicAdd(&s_cnt[cit], 1u);
+    else rk = atomicAdd(&cell_count[((size_t)cx * L.ncy + cy) * L.ncz + cz], 1u);
+    rk = rk < rank_max ? rk : rank_max;
+    key[base + r] = (tile << (L.rank_bits + L.cell_bits)) | (rk << L.cell_bits) | cit;
+    idx[base + r] = (uint32_t)j;
   }
 }
 
@@ -404,10 +458,15 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
     hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(ncell)), dim3(256), 0, s, cell_count,
                        ncell);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_keys<T>, dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
-                       (const T *)src[0], (const T *)src[1], (const T *)src[2],
-                       binned ? (const unsigned int *)P : nullptr, src_seg, src_cnt,
-                       src_ovf_cap, cell_count, keys_in, idx_in);
+    if (binned)
+      hipLaunchKernelGGL(k_bin_keys_binned<T>, dim3((unsigned)nt1), dim3(256), 0, s, g, L,
+                         (const T *)src[0], (const T *)src[1], (const T *)src[2],
+                         (const unsigned int *)P, src_seg, src_cnt, src_ovf_cap, cell_count,
+                         keys_in, idx_in);
+    else
+      hipLaunchKernelGGL(k_bin_keys<T>, dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
+                         (const T *)src[0], (const T *)src[1], (const T *)src[2],
+                         cell_count, keys_in, idx_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, idx_in,
                                   perm, (size_t)n, 0, key_bits, s);
