@@ -252,13 +252,14 @@ STRESS_CYCLES = 50
 STRESS_SORT_EVERY = 5
 
 
-def run_stress(warmup: int) -> dict:
+def run_stress(warmup: int, sort_mode: int = 0) -> dict:
     """The combined stress shape: the 128x64x64 Harris sheet with its four
     species at 5x5x5 per cell plus four uniform beams (electrons and ions,
     x drift +-0.1, thermal speed 0.02) at 1x5x5 = 25 per cell, 314,572,800
     particles in FP32; the engine rebuilds its tile segments every 5 phases;
     50 cycles (engine phases that write the moments) are captured in one HIP graph
-    and timed as one launch."""
+    and timed as one launch.  sort_mode 1 makes three of every four re-sorts
+    an in-tile reorder."""
     import ctypes as C
 
     import sputnipic_amd as S
@@ -270,7 +271,8 @@ def run_stress(warmup: int) -> dict:
     field = S.gem_fields(grd, gem)
     species = S.harris_species(PPC) + S.beam_species()
     ns = len(species)
-    eng = S.Engine(grd, ns, sort_every=STRESS_SORT_EVERY, set_moments=True)
+    eng = S.Engine(grd, ns, sort_every=STRESS_SORT_EVERY, set_moments=True,
+                   sort_mode=sort_mode)
     stream = S.Stream()
     n_total = 0
     for s, sp in enumerate(species):
@@ -280,8 +282,8 @@ def run_stress(warmup: int) -> dict:
         stream.synchronize()
         for d in p.d.values():
             d.free()
-    log(f"stress: {n_total} particles in {ns} species, FP32, segments rebuilt every "
-        f"{STRESS_SORT_EVERY} phases")
+    log(f"stress: {n_total} particles in {ns} species, FP32, re-sorted every "
+        f"{STRESS_SORT_EVERY} phases, sort mode {sort_mode}")
     moms = [S.InterpDensSpecies(grd) for _ in range(ns)]
     param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
 
@@ -788,6 +790,9 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
+    ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=0,
+                    help="stress: 0 full re-sorts, 1 in-tile reorders with every fourth "
+                         "re-sort a full one")
     ap.add_argument("--workload", choices=("gem", "stress", "batch", "field"), default="gem",
                     help="gem: the 262M-particle Harris sheet (default); stress: 8 "
                          "species, 314.6M particles, FP32, 50 cycles in one HIP graph; "
@@ -824,7 +829,7 @@ def main() -> int:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
     if args.workload == "stress":
-        f = run_stress(args.warmup)
+        f = run_stress(args.warmup, args.sort_mode)
         line = {
             "metric": METRIC,
             "value": f["value"],
@@ -844,6 +849,7 @@ def main() -> int:
                             "per cell = 314,572,800 particles, FP32, tile segments rebuilt "
                             "every 5 phases, 50 cycles captured in one HIP graph",
                 "particles": f["particles"],
+                "sort_mode": args.sort_mode,
             },
             "roofline": {
                 "bound": "hbm",

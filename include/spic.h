@@ -265,6 +265,12 @@ typedef struct spic_engine_opts {
                                  (src/sputniPIC_CPU.cpp:247; rhoc is not
                                  touched); 0 (default): ADD them, as
                                  interpP2G does */
+  int sort_mode;              /* re-sorts (sort_every, spic_engine_sort):
+                                 0 (default) a full sort that also re-sizes
+                                 the segments to the current counts; 1 an
+                                 in-tile reorder (each tile's particles in
+                                 round-robin cell order, segments kept),
+                                 with every fourth re-sort a full one */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {
@@ -321,7 +327,7 @@ int spic_engine_get_stats(const spic_engine *eng, spic_engine_stats *st);
 /* are one launch each.  The reference runs independent simulations as    */
 /* separate processes; this is the same work in one stream of launches.    */
 /* Each instance's results equal those of spic_engine_phase on its own.    */
-/* Members share precision, tile shape and LDS partials mode.  The field,  */
+/* Members share precision, tile shape, LDS partials and sort mode.  The  */
 /* moment and push descriptors are copied at creation (the arrays they     */
 /* point to must stay allocated).  Load every species before creating the  */
 /* batch; afterwards advance the members only through spic_batch_phase     */

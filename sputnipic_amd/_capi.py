@@ -93,7 +93,7 @@ class spic_gem_species(C.Structure):
 class spic_engine_opts(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
                 ("sort_every", C.c_int), ("lds_partials", C.c_int),
-                ("slack_pct", C.c_int), ("set_moments", C.c_int)]
+                ("slack_pct", C.c_int), ("set_moments", C.c_int), ("sort_mode", C.c_int)]
 
 
 class spic_engine_stats(C.Structure):

@@ -613,13 +613,14 @@ class Engine:
     interpP2G in one launch over HBM-resident particles binned by tile."""
 
     def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
-                 lds_partials: int = 0, slack_pct: int = 0, set_moments: bool = False):
+                 lds_partials: int = 0, slack_pct: int = 0, set_moments: bool = False,
+                 sort_mode: int = 0):
         from ._capi import spic_engine_opts
         self.grd = grd
         self.ns = ns
         p = C.c_void_p()
         opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials,
-                                slack_pct, int(bool(set_moments)))
+                                slack_pct, int(bool(set_moments)), int(sort_mode))
         g = grd.c_struct()
         check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
               "spic_engine_create")

@@ -219,6 +219,17 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, T *const dst[7], hipStream_t s);
 
+// In-tile reorder of one species from buffer src to buffer dst in the same
+// segment layout (seg): each tile's particles in round-robin cell order;
+// particles outside their tile's cells, and the overflow region's, go to
+// dst's overflow region.  Component c of slot i at base[c * stride + i].
+// dst_cnt (3 * (ntiles + 1) words) must be zero on entry.
+template <typename T>
+hipError_t tile_reorder(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
+                        const uint32_t *src_id, const int64_t *seg,
+                        const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
+                        uint32_t *dst_id, unsigned int *dst_cnt, hipStream_t s);
+
 // bin_store of x y z u v w only, as big-endian IEEE float32 words (values
 // rounded to float): the particle datasets of a snapshot.
 template <typename T>

@@ -398,6 +398,124 @@ template hipError_t launch_batch_zero_counts<double>(const BinnedArgs<double> *,
 template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
                                                     const BatchMap &, int, int, hipStream_t);
 
+// In-tile reorder: block t < ntiles takes tile t's segment of the source
+// buffer and writes it to the same segment of the destination buffer in
+// round-robin cell order -- rank 0 of every cell in cell order, then rank 1,
+// and so on, the order a full sort gives -- by a counting sort in LDS:
+// pass 1 counts the particles of each cell, LDS tables give the slot of the
+// k-th particle of cell c as A[k] + T[k][c] (A[k]: particles of rank < k
+// over all cells; T[k][c]: cells before c holding more than k particles),
+// pass 2 scatters.  Ranks at or beyond the tables' KT rows take the slot by
+// a loop over the cells.  A particle whose cell lies in another tile, and
+// every particle of the overflow region (block ntiles), goes to the
+// destination's overflow region; the next phase pushes it from there.
+// dst_cnt must be zero on entry; this writes its per-tile counts.
+template <typename T, int NC, int KT>
+__global__ __launch_bounds__(256) void k_tile_reorder(
+    KGrid<T> g, BinLayout L, const T *__restrict__ src, int64_t stride,
+    const uint32_t *__restrict__ src_id, const int64_t *__restrict__ seg,
+    const unsigned int *__restrict__ src_cnt, unsigned int ovf_cap, T *__restrict__ dst,
+    uint32_t *__restrict__ dst_id, unsigned int *__restrict__ dst_cnt) {
+  __shared__ unsigned int s_count[NC], s_rank[NC], s_kmax;
+  __shared__ unsigned int s_A[KT];
+  __shared__ unsigned char s_T[KT][NC];
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  const RangeView v = range_view(seg, src_cnt, L.ntiles, ovf_cap, t);
+  const bool own_tile = t < L.ntiles;
+  for (int c = tid; c < NC; c += 256) {
+    s_count[c] = 0u;
+    s_rank[c] = 0u;
+  }
+  if (tid == 0) s_kmax = 0u;
+  __syncthreads();
+  if (own_tile) {
+    for (int64_t r = tid; r < v.count; r += 256) {
+      const int64_t j = v.slot(r);
+      int cx, cy, cz;
+      uint32_t tile, cit;
+      cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile, cit);
+      if ((int)tile == t) atomicAdd(&s_count[cit], 1u);
+    }
+    __syncthreads();
+    for (int c = tid; c < NC; c += 256) atomicMax(&s_kmax, s_count[c]);
+    __syncthreads();
+    const int kmax = (int)(s_kmax < (unsigned)KT ? s_kmax : (unsigned)KT);
+    for (int k = tid; k < kmax; k += 256) {
+      unsigned int a = 0, before = 0;
+      for (int c = 0; c < NC; ++c) {
+        const unsigned int n = s_count[c];
+        a += n < (unsigned)k ? n : (unsigned)k;
+        s_T[k][c] = (unsigned char)before;
+        before += n > (unsigned)k ? 1u : 0u;
+      }
+      s_A[k] = a;
+    }
+    if (tid == 0) {
+      unsigned int own = 0;
+      for (int c = 0; c < NC; ++c) own += s_count[c];
+      dst_cnt[t] = own;
+    }
+    __syncthreads();
+  }
+  for (int64_t r = tid; r < v.count; r += 256) {
+    const int64_t j = v.slot(r);
+    int cx, cy, cz;
+    uint32_t tile, cit;
+    cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile, cit);
+    int64_t o;
+    if (own_tile && (int)tile == t) {
+      const unsigned int k = atomicAdd(&s_rank[cit], 1u);
+      unsigned int pos;
+      if (k < (unsigned)KT) {
+        pos = s_A[k] + s_T[k][cit];
+      } else {
+        pos = 0;
+        for (int c = 0; c < NC; ++c) {
+          const unsigned int n = s_count[c];
+          pos += n < k ? n : k;
+          if (c < (int)cit && n > k) ++pos;
+        }
+      }
+      o = seg[t] + pos;
+    } else {
+      const unsigned int k = atomicAdd(&dst_cnt[L.ntiles], 1u);
+      if (k >= ovf_cap) continue;  // cannot happen: the region holds every particle
+      o = seg[L.ntiles] + k;
+    }
+#pragma unroll
+    for (int a = 0; a < 7; ++a) dst[a * stride + o] = src[a * stride + j];
+    dst_id[o] = src_id[j];
+  }
+}
+
+template <typename T>
+hipError_t tile_reorder(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
+                        const uint32_t *src_id, const int64_t *seg,
+                        const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
+                        uint32_t *dst_id, unsigned int *dst_cnt, hipStream_t s) {
+  const int nc = L.tx * L.ty * L.tz;
+  const dim3 grid(L.ntiles + 1);
+  if (nc == 64)
+    hipLaunchKernelGGL((k_tile_reorder<T, 64, 512>), grid, dim3(256), 0, s, g, L, src, stride,
+                       src_id, seg, src_cnt, ovf_cap, dst, dst_id, dst_cnt);
+  else if (nc == 128)
+    hipLaunchKernelGGL((k_tile_reorder<T, 128, 256>), grid, dim3(256), 0, s, g, L, src,
+                       stride, src_id, seg, src_cnt, ovf_cap, dst, dst_id, dst_cnt);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template hipError_t tile_reorder<double>(const KGrid<double> &, const BinLayout &,
+                                         const double *, int64_t, const uint32_t *,
+                                         const int64_t *, const unsigned int *, unsigned int,
+                                         double *, uint32_t *, unsigned int *, hipStream_t);
+template hipError_t tile_reorder<float>(const KGrid<float> &, const BinLayout &,
+                                        const float *, int64_t, const uint32_t *,
+                                        const int64_t *, const unsigned int *, unsigned int,
+                                        float *, uint32_t *, unsigned int *, hipStream_t);
+
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_counts, dim3(nblocks(c.n > c.nn ? c.n : c.nn), c.ns), dim3(256),

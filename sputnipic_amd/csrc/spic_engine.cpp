@@ -23,6 +23,7 @@ struct spic_engine {
   spic::BinLayout L{};
   int slack_pct = 12;
   int sort_every = 0;
+  int sort_mode = 0;               // 0 full sort; 1 in-tile reorder (see opts)
   int lds_fp64 = 1;
   int set_moments = 0;             // phases write the moments (else add)
   int64_t phases = 0, sorts = 0;
@@ -43,6 +44,7 @@ struct spic_engine {
     int64_t *seg[2] = {nullptr, nullptr};       // [ntiles + 1] per layout
     int cur = 0;                   // buffer holding the particles
     int segi = 0;                  // layout in use
+    int64_t resorts = 0;           // sorts since load
   } sp[spic::kMaxSpecies];
   unsigned long long *counters = nullptr;  // see BinnedArgs::counters
   void *partials = nullptr;        // see BinnedArgs::partials
@@ -95,11 +97,32 @@ hipError_t ensure_workspace(spic_engine *e, size_t need, hipStream_t s) {
   return hipSuccess;
 }
 
+// In-tile reorder of species `is` into the other buffer, same layout.
+template <typename T>
+hipError_t reorder_species(spic_engine *e, int is, hipStream_t s) {
+  auto &sp = e->sp[is];
+  const int from = sp.cur, to = 1 - sp.cur;
+  const size_t ncnt = 3 * (size_t)(e->L.ntiles + 1);
+  hipError_t err = spic::zero_u32(sp.cnt[to], ncnt, s);
+  if (err != hipSuccess) return err;
+  err = spic::tile_reorder<T>(spic::kgrid<T>(&e->grd), e->L, (const T *)sp.block[from],
+                              sp.cap, sp.id[from], sp.seg[sp.segi], sp.cnt[from],
+                              sp.ovf_cap, (T *)sp.block[to], sp.id[to], sp.cnt[to], s);
+  if (err != hipSuccess) return err;
+  sp.cur = to;
+  e->sorts++;
+  return hipSuccess;
+}
+
 // Sort species `is` into the other buffer and the other layout.  With
 // `src` given, the particles come densely from the caller's arrays instead.
+// In sort mode 1 a re-sort is an in-tile reorder, except every fourth,
+// which re-sizes the segments to the current counts.
 template <typename T>
 hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
   auto &sp = e->sp[is];
+  if (src) sp.resorts = 0;
+  else if (e->sort_mode == 1 && (sp.resorts++ % 4) != 3) return reorder_species<T>(e, is, s);
   const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
   spic::BinLayout L = e->L;
   L.slack = sp.slack;
@@ -236,7 +259,8 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (rc) return rc;
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
-  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0;
+  int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0,
+      sort_mode = 0;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -251,6 +275,9 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->set_moments != 0 && opts->set_moments != 1)
       return fail(SPIC_EINVAL, "set_moments must be 0 or 1");
     set_mom = opts->set_moments;
+    if (opts->sort_mode != 0 && opts->sort_mode != 1)
+      return fail(SPIC_EINVAL, "sort_mode must be 0 or 1");
+    sort_mode = opts->sort_mode;
   }
   // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
   if (grd->nxn > 4096 || grd->nyn > 1024 || grd->nzn > 1024)
@@ -266,6 +293,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->ns = ns;
   e->esz = grd->prec == SPIC_F64 ? 8 : 4;
   e->sort_every = sort_every;
+  e->sort_mode = sort_mode;
   e->lds_fp64 = lds_fp64;
   e->slack_pct = slack_pct;
   e->set_moments = set_mom;
@@ -617,10 +645,10 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
     if ((rc = spic::check_field(m.field)) || (rc = spic::check_push(m.push))) return rc;
     const spic_engine *e = m.engine, *e0 = members[0].engine;
     if (e->grd.prec != e0->grd.prec || e->L.tx != e0->L.tx || e->L.ty != e0->L.ty ||
-        e->L.tz != e0->L.tz || e->lds_fp64 != e0->lds_fp64)
+        e->L.tz != e0->L.tz || e->lds_fp64 != e0->lds_fp64 || e->sort_mode != e0->sort_mode)
       return fail(SPIC_EINVAL,
-                  "batch member %d: precision, tile shape or LDS partials differ from "
-                  "member 0's", i);
+                  "batch member %d: precision, tile shape, LDS partials or sort mode "
+                  "differ from member 0's", i);
     for (int j = 0; j < i; ++j)
       if (members[j].engine == e)
         return fail(SPIC_EINVAL, "batch members %d and %d share an engine", j, i);
@@ -697,16 +725,27 @@ int spic_batch_phase(spic_batch *b, void *stream) {
     if (err != hipSuccess) return hip_fail(err, "batch argument tables");
   }
   if (sort_now) {
+    // A full sort moves to the other buffer and the other layout, an
+    // in-tile reorder to the other buffer only; members sorted in step
+    // land in one state.
+    int after = -1;
     for (int i = 0; i < b->n; ++i) {
       spic_engine *e = b->e[i];
       std::lock_guard<std::mutex> lock(e->mu);
       for (int is = 0; is < e->ns; ++is) {
-        if (!e->sp[is].loaded || e->sp[is].n == 0) continue;
+        auto &sp = e->sp[is];
+        if (!sp.loaded || sp.n == 0) continue;
         if ((err = sort_any(e, is, nullptr, s)) != hipSuccess)
           return hip_fail(err, "batch sort");
+        const int st2 = sp.segi * 2 + sp.cur;
+        if (after < 0) after = st2;
+        else if (st2 != after)
+          return fail(SPIC_EINVAL,
+                      "batch members out of step after a sort (member %d species %d): "
+                      "their sort modes or sort histories differ", i, is);
       }
     }
-    state ^= 3;  // a sort moves to the other buffer and the other layout
+    state = after;
   }
   err = b->prec == SPIC_F64 ? launch_batch<double>(b, state, s)
                             : launch_batch<float>(b, state, s);

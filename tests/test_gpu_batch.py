@@ -47,7 +47,7 @@ def _oracle(spec, prec):
 
 
 class Instance:
-    def __init__(self, st, prec, tile=(4, 4, 4), lds_partials=0):
+    def __init__(self, st, prec, tile=(4, 4, 4), lds_partials=0, sort_mode=0):
         c = st.cfg
         dt = np.float64 if prec == "f64" else np.float32
         self.grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
@@ -58,7 +58,8 @@ class Instance:
         self.param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover,
                                   PERIODICX=c.PERIODICX, PERIODICY=c.PERIODICY,
                                   PERIODICZ=c.PERIODICZ)
-        self.eng = S.Engine(self.grd, c.ns, tile=tile, lds_partials=lds_partials)
+        self.eng = S.Engine(self.grd, c.ns, tile=tile, lds_partials=lds_partials,
+                            sort_mode=sort_mode)
         for s, p in enumerate(self.parts):
             self.eng.load(s, p)
         self.ns = c.ns
@@ -105,6 +106,25 @@ def test_batch_matches_oracle_per_instance(prec, tile, sort_every):
         assert stats["phases"] == phases
         if sort_every:
             assert stats["sorts"] == x.ns * (1 + (phases - 1) // sort_every)
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_in_tile_reorder_matches_oracle(prec):
+    """Members in sort mode 1, re-sorted every phase for six phases: the
+    batch follows the members' buffer and layout through in-tile reorders
+    (buffer only) and the fourth re-sort (buffer and layout)."""
+    states = [_oracle(spec, prec) for spec in INSTANCES]
+    insts = [Instance(st, prec, sort_mode=1) for st in states]
+    batch = S.Batch([x.member() for x in insts], sort_every=1, set_moments=True)
+    for _ in range(6):
+        batch.phase()
+    for st in states:
+        for _ in range(6):
+            st.cycle()
+    _check([x.results() for x in insts], states, prec, "sort mode 1")
     batch.close()
     for x in insts:
         x.eng.close()
@@ -208,6 +228,10 @@ def test_batch_rejects_bad_members():
     with pytest.raises(S.SpicError) as ei:
         S.Batch([a.member(), c.member()])
     assert ei.value.code == S.SPIC_EINVAL
+    e = Instance(_oracle(INSTANCES[1], "f64"), "f64", sort_mode=1)
+    with pytest.raises(S.SpicError) as ei:
+        S.Batch([a.member(), e.member()])
+    assert ei.value.code == S.SPIC_EINVAL
     # a member advanced on its own falls out of step with the others
     d = Instance(_oracle(INSTANCES[2], "f64"), "f64")
     batch = S.Batch([a.member(), d.member()])
@@ -217,7 +241,7 @@ def test_batch_rejects_bad_members():
         batch.phase()
     assert ei.value.code == S.SPIC_EINVAL
     batch.close()
-    for x in (a, b, c, d):
+    for x in (a, b, c, d, e):
         x.eng.close()
 
 

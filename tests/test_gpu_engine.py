@@ -43,7 +43,7 @@ def oracle_small(prec):
 
 
 def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0,
-               slack_pct=0, set_moments=False):
+               slack_pct=0, set_moments=False, sort_mode=0):
     c = st.cfg
     grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
     dt = np.float64 if prec == "f64" else np.float32
@@ -53,7 +53,7 @@ def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0
     param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
                          PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
     eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every, slack_pct=slack_pct,
-                   set_moments=set_moments)
+                   set_moments=set_moments, sort_mode=sort_mode)
     for s, p in enumerate(parts):
         eng.load(s, p)
     if set_moments:
@@ -121,6 +121,54 @@ def test_engine_tile_shapes_and_sorting_do_not_change_trajectories(tile, sort_ev
             assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
         for n in O.MOMENT_NAMES:
             assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("tile", [(4, 4, 4), (8, 4, 4)])
+def test_engine_in_tile_reorder_keeps_trajectories(prec, tile):
+    """sort_mode 1, a re-sort every phase over seven phases: re-sorts 1-3
+    and 5-6 are in-tile reorders, the fourth a full sort.  Particles, ids
+    and counters pass through both kinds bit-exactly."""
+    st = oracle_small(prec)
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, prec, phases=7, tile=tile, sort_every=1,
+                                     sort_mode=1)
+    oracle_phases(st, 7)
+    assert stats["sorts"] == st.cfg.ns * 7
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
+def test_engine_in_tile_reorder_moves_foreign_particles_to_overflow():
+    """gem_hot without E (speeds below a cell per step) re-sorted every
+    phase in sort mode 1: particles that left their tile since the last
+    full sort go to the overflow region, and the next phase still finds
+    and moves them exactly."""
+    data = load_case("gem_hot", "f64")
+    st = oracle_from_fixture("gem_hot", "f64", data)
+    for n in ("Ex", "Ey", "Ez"):
+        st.fields[n][:] = 0.0
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, "f64", phases=6, sort_every=1, sort_mode=1)
+    oracle_phases(st, 6)
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
+
+
+def test_engine_rejects_unknown_sort_mode():
+    st = oracle_small("f64")
+    c = st.cfg
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f64")
+    with pytest.raises(S.SpicError) as ei:
+        S.Engine(grd, c.ns, sort_mode=2)
+    assert ei.value.code == S.SPIC_EINVAL
 
 
 @pytest.mark.parametrize("prec", PRECS)
