@@ -258,8 +258,8 @@ def run_stress(warmup: int, sort_mode: int = 0) -> dict:
     x drift +-0.1, thermal speed 0.02) at 1x5x5 = 25 per cell, 314,572,800
     particles in FP32; the engine rebuilds its tile segments every 5 phases;
     50 cycles (engine phases that write the moments) are captured in one HIP graph
-    and timed as one launch.  sort_mode 1 makes three of every four re-sorts
-    an in-tile reorder."""
+    and timed as one launch.  Re-sorts are tile sorts (sort_mode 0) or radix
+    sorts (1)."""
     import ctypes as C
 
     import sputnipic_amd as S
@@ -791,8 +791,7 @@ def main() -> int:
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
     ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=0,
-                    help="stress: 0 full re-sorts, 1 in-tile reorders with every fourth "
-                         "re-sort a full one")
+                    help="stress: re-sorts by tile sort (0) or radix sort (1)")
     ap.add_argument("--workload", choices=("gem", "stress", "batch", "field"), default="gem",
                     help="gem: the 262M-particle Harris sheet (default); stress: 8 "
                          "species, 314.6M particles, FP32, 50 cycles in one HIP graph; "

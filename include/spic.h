@@ -246,6 +246,9 @@ int spic_maxwell_image(const spic_grid *grd, const spic_field *field,
 /* ---------------------------------------------------------------------- */
 typedef struct spic_engine spic_engine;
 
+#define SPIC_SORT_TILE 0
+#define SPIC_SORT_RADIX 1
+
 typedef struct spic_engine_opts {
   int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s) or 8x4x4 */
   int sort_every;             /* rebuild the segments every N phases; <= 0:
@@ -266,11 +269,15 @@ typedef struct spic_engine_opts {
                                  touched); 0 (default): ADD them, as
                                  interpP2G does */
   int sort_mode;              /* re-sorts (sort_every, spic_engine_sort):
-                                 0 (default) a full sort that also re-sizes
-                                 the segments to the current counts; 1 an
-                                 in-tile reorder (each tile's particles in
-                                 round-robin cell order, segments kept),
-                                 with every fourth re-sort a full one */
+                                 SPIC_SORT_TILE (0, default) counts within
+                                 each tile; SPIC_SORT_RADIX a radix sort
+                                 of all particles by (tile, rank within
+                                 cell, cell).  Both give the same segments
+                                 and the same order within each tile,
+                                 except that the tile sort puts particles
+                                 arriving from other tiles after the
+                                 tile's own.  Trajectories do not depend
+                                 on the mode; loads always radix sort */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {

@@ -21,6 +21,8 @@ SPIC_EUNSUPPORTED = 4
 SPIC_ENOMEM = 5
 SPIC_F64 = 0
 SPIC_F32 = 1
+SPIC_SORT_TILE = 0
+SPIC_SORT_RADIX = 1
 
 STATUS_NAMES = {
     SPIC_OK: "SPIC_OK", SPIC_EINVAL: "SPIC_EINVAL", SPIC_EHIP: "SPIC_EHIP",

@@ -219,16 +219,21 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, T *const dst[7], hipStream_t s);
 
-// In-tile reorder of one species from buffer src to buffer dst in the same
-// segment layout (seg): each tile's particles in round-robin cell order;
-// particles outside their tile's cells, and the overflow region's, go to
-// dst's overflow region.  Component c of slot i at base[c * stride + i].
-// dst_cnt (3 * (ntiles + 1) words) must be zero on entry.
+// Tile sort of one binned species from buffer src (layout src_seg) to
+// buffer dst (layout dst_seg): the binned source's alternative to bin_sort,
+// by counting within tiles instead of a radix sort over all particles.
+// The destination gets bin_sort's segments (count + slack slots each, an
+// empty overflow region); each tile holds its own particles (those that
+// were in its segment and are in its cells) in round-robin cell order --
+// bin_sort's order -- followed by the particles that came from other
+// segments or the overflow region.  Component c of slot i at
+// base[c * stride + i].  With workspace == nullptr only *needed is set.
 template <typename T>
-hipError_t tile_reorder(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
-                        const uint32_t *src_id, const int64_t *seg,
-                        const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
-                        uint32_t *dst_id, unsigned int *dst_cnt, hipStream_t s);
+hipError_t tile_sort(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
+                     const uint32_t *src_id, const int64_t *src_seg,
+                     const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
+                     uint32_t *dst_id, int64_t *dst_seg, unsigned int *dst_cnt,
+                     void *workspace, size_t workspace_bytes, size_t *needed, hipStream_t s);
 
 // bin_store of x y z u v w only, as big-endian IEEE float32 words (values
 // rounded to float): the particle datasets of a snapshot.

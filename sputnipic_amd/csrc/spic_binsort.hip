@@ -398,123 +398,358 @@ template hipError_t launch_batch_zero_counts<double>(const BinnedArgs<double> *,
 template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
                                                     const BatchMap &, int, int, hipStream_t);
 
-// In-tile reorder: block t < ntiles takes tile t's segment of the source
-// buffer and writes it to the same segment of the destination buffer in
-// round-robin cell order -- rank 0 of every cell in cell order, then rank 1,
-// and so on, the order a full sort gives -- by a counting sort in LDS:
-// pass 1 counts the particles of each cell, LDS tables give the slot of the
-// k-th particle of cell c as A[k] + T[k][c] (A[k]: particles of rank < k
-// over all cells; T[k][c]: cells before c holding more than k particles),
-// pass 2 scatters.  Ranks at or beyond the tables' KT rows take the slot by
-// a loop over the cells.  A particle whose cell lies in another tile, and
-// every particle of the overflow region (block ntiles), goes to the
-// destination's overflow region; the next phase pushes it from there.
-// dst_cnt must be zero on entry; this writes its per-tile counts.
-template <typename T, int NC, int KT>
-__global__ __launch_bounds__(256) void k_tile_reorder(
+// Copy particle j of the source to slot o of the destination.
+template <typename T>
+__device__ __forceinline__ void copy_particle(const T *__restrict__ src,
+                                              const uint32_t *__restrict__ src_id,
+                                              int64_t stride, int64_t j, T *__restrict__ dst,
+                                              uint32_t *__restrict__ dst_id, int64_t o) {
+#pragma unroll
+  for (int a = 0; a < 7; ++a) dst[a * stride + o] = src[a * stride + j];
+  dst_id[o] = src_id[j];
+}
+
+constexpr int kTileThreads = 512;
+constexpr int kTileOvfBlocks = 64;          // blocks that split the overflow region
+constexpr unsigned int kTileStaged = 12288;  // particles a tile block stages in LDS
+constexpr int kTileStageBytes = 32768;       // one output chunk of one component
+constexpr int kTileU = 4;                    // particles a thread loads at once
+
+// Tile sort, step 1.  Block t < ntiles reads segment t: its particles that
+// lie in tile t are tile t's own (own[t], added to count[t]); every other
+// particle adds one to its tile's count.  Blocks ntiles and up split the
+// overflow region, whose particles all add one to their tile's count.
+template <typename T>
+__global__ __launch_bounds__(kTileThreads) void k_tile_count(
     KGrid<T> g, BinLayout L, const T *__restrict__ src, int64_t stride,
-    const uint32_t *__restrict__ src_id, const int64_t *__restrict__ seg,
+    const int64_t *__restrict__ seg, const unsigned int *__restrict__ cnt,
+    unsigned int ovf_cap, unsigned int *__restrict__ count, unsigned int *__restrict__ own) {
+  __shared__ unsigned int s_own;
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const bool ovf = t >= L.ntiles;
+  const RangeView v = range_view(seg, cnt, L.ntiles, ovf_cap, ovf ? L.ntiles : t);
+  const int64_t first = ovf ? (int64_t)(t - L.ntiles) * kTileThreads : 0;
+  const int64_t step =
+      ovf ? (int64_t)(gridDim.x - L.ntiles) * kTileThreads : (int64_t)kTileThreads;
+  if (tid == 0) s_own = 0u;
+  __syncthreads();
+  unsigned int mine = 0;
+  for (int64_t base = first; base < v.count; base += kTileU * step) {
+    T px[kTileU], py[kTileU], pz[kTileU];
+#pragma unroll
+    for (int u = 0; u < kTileU; ++u) {
+      const int64_t r = base + u * step + tid;
+      if (r < v.count) {
+        const int64_t j = v.slot(r);
+        px[u] = src[j];
+        py[u] = src[stride + j];
+        pz[u] = src[2 * stride + j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTileU; ++u) {
+      const bool in = base + u * step + tid < v.count;
+      uint32_t tile = 0;
+      if (in) {
+        int cx, cy, cz;
+        uint32_t cit;
+        cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
+      }
+      const bool home = in && !ovf && (int)tile == t;
+      mine += home ? 1u : 0u;
+      (void)wave_rank(count, tile, in && !home);
+    }
+  }
+  if (ovf) return;
+  if (mine) atomicAdd(&s_own, mine);
+  __syncthreads();
+  if (tid == 0) {
+    own[t] = s_own;
+    if (s_own) atomicAdd(&count[t], s_own);
+  }
+}
+
+// Tile sort, step 2: the destination layout from the exclusive scan P of
+// the counts (P[ntiles] = n): seg[t] = P[t] + t * slack, cnt[t] = count[t],
+// no unused runs, an empty overflow region.
+__global__ void k_tile_bounds(const unsigned int *__restrict__ P,
+                              const unsigned int *__restrict__ count, int ntiles, int64_t slack,
+                              int64_t *__restrict__ seg, unsigned int *__restrict__ cnt) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
+       t += gridDim.x * blockDim.x) {
+    seg[t] = (int64_t)P[t] + (int64_t)t * slack;
+    cnt[t] = t < ntiles ? count[t] : 0u;
+    cnt[(ntiles + 1) + t] = 0u;
+    cnt[2 * (ntiles + 1) + t] = 0u;
+  }
+}
+
+// Tile sort, step 3.  Block t < ntiles writes tile t's own particles to
+// the start of its destination segment in round-robin cell order -- rank 0
+// of every cell in cell order, then rank 1, and so on: bin_sort's order --
+// and every other particle of its source segment after the own particles
+// of that particle's tile (slot dst_seg[tile] + own[tile] + k, k from the
+// fill counters).  Blocks ntiles and up do the latter for the overflow
+// region.  For the own particles:
+//  1. each particle's cell within the tile (or "foreign") goes to LDS, and
+//     LDS counters count each cell;
+//  2. tables give the slot of the k-th particle of cell c as A[k] + T[c][k]
+//     (A[k]: particles of rank < k over all cells; T[c][k]: cells before c
+//     holding more than k particles); ranks beyond the tables' KT columns
+//     take the slot by a loop over the cells; each particle's slot goes to
+//     LDS;
+//  3. per component and output chunk of the LDS stage, the chunk's
+//     particles are read in source order (coalesced) into the stage at their
+//     slots, and the stage is written out in slot order (coalesced).
+// A segment of more than kTileStaged particles skips the stage and writes
+// each particle at its slot directly.
+template <typename T, int NC, int KT>
+__global__ __launch_bounds__(kTileThreads) void k_tile_sort(
+    KGrid<T> g, BinLayout L, const T *__restrict__ src, int64_t stride,
+    const uint32_t *__restrict__ src_id, const int64_t *__restrict__ src_seg,
     const unsigned int *__restrict__ src_cnt, unsigned int ovf_cap, T *__restrict__ dst,
-    uint32_t *__restrict__ dst_id, unsigned int *__restrict__ dst_cnt) {
-  __shared__ unsigned int s_count[NC], s_rank[NC], s_kmax;
+    uint32_t *__restrict__ dst_id, const int64_t *__restrict__ dst_seg,
+    const unsigned int *__restrict__ own, unsigned int *__restrict__ fill) {
+  static_assert(KT * NC <= kTileStageBytes, "slot table exceeds the stage");
+  static_assert(NC <= 255, "cell index must fit a byte below the foreign mark");
+  constexpr unsigned char kForeign = 0xFF;
+  constexpr uint16_t kNoSlot = 0xFFFF;
+  constexpr int kChunk = kTileStageBytes / (int)sizeof(T);
+  __shared__ unsigned int s_count[NC], s_rank[NC], s_kmax, s_own;
   __shared__ unsigned int s_A[KT];
-  __shared__ unsigned char s_T[KT][NC];
+  __shared__ uint16_t s_slot[kTileStaged];
+  __shared__ unsigned char s_cell[kTileStaged];
+  __shared__ __align__(16) unsigned char s_raw[kTileStageBytes];
+  unsigned char(*s_T)[KT] = reinterpret_cast<unsigned char(*)[KT]>(s_raw);
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
-  const RangeView v = range_view(seg, src_cnt, L.ntiles, ovf_cap, t);
-  const bool own_tile = t < L.ntiles;
-  for (int c = tid; c < NC; c += 256) {
+
+  if (t >= L.ntiles) {
+    const RangeView v = range_view(src_seg, src_cnt, L.ntiles, ovf_cap, L.ntiles);
+    const int64_t step = (int64_t)(gridDim.x - L.ntiles) * kTileThreads;
+    for (int64_t base = (int64_t)(t - L.ntiles) * kTileThreads; base < v.count;
+         base += step) {
+      const int64_t r = base + tid;
+      const bool in = r < v.count;
+      int64_t j = 0;
+      uint32_t tile = 0;
+      if (in) {
+        j = v.slot(r);
+        int cx, cy, cz;
+        uint32_t cit;
+        cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
+                   cit);
+      }
+      const uint32_t k = wave_rank(fill, tile, in);
+      if (in) copy_particle(src, src_id, stride, j, dst, dst_id, dst_seg[tile] + own[tile] + k);
+    }
+    return;
+  }
+
+  const RangeView v = range_view(src_seg, src_cnt, L.ntiles, ovf_cap, t);
+  if (v.count == 0) return;
+  const bool staged = v.count <= (int64_t)kTileStaged;
+  for (int c = tid; c < NC; c += kTileThreads) {
     s_count[c] = 0u;
     s_rank[c] = 0u;
   }
   if (tid == 0) s_kmax = 0u;
   __syncthreads();
-  if (own_tile) {
-    for (int64_t r = tid; r < v.count; r += 256) {
-      const int64_t j = v.slot(r);
-      int cx, cy, cz;
-      uint32_t tile, cit;
-      cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile, cit);
-      if ((int)tile == t) atomicAdd(&s_count[cit], 1u);
-    }
-    __syncthreads();
-    for (int c = tid; c < NC; c += 256) atomicMax(&s_kmax, s_count[c]);
-    __syncthreads();
-    const int kmax = (int)(s_kmax < (unsigned)KT ? s_kmax : (unsigned)KT);
-    for (int k = tid; k < kmax; k += 256) {
-      unsigned int a = 0, before = 0;
-      for (int c = 0; c < NC; ++c) {
-        const unsigned int n = s_count[c];
-        a += n < (unsigned)k ? n : (unsigned)k;
-        s_T[k][c] = (unsigned char)before;
-        before += n > (unsigned)k ? 1u : 0u;
+
+  // 1. cells and counts
+  for (int64_t r0 = tid; r0 < v.count; r0 += kTileU * kTileThreads) {
+    T px[kTileU], py[kTileU], pz[kTileU];
+#pragma unroll
+    for (int u = 0; u < kTileU; ++u) {
+      const int64_t r = r0 + (int64_t)u * kTileThreads;
+      if (r < v.count) {
+        const int64_t j = v.slot(r);
+        px[u] = src[j];
+        py[u] = src[stride + j];
+        pz[u] = src[2 * stride + j];
       }
-      s_A[k] = a;
-    }
-    if (tid == 0) {
-      unsigned int own = 0;
-      for (int c = 0; c < NC; ++c) own += s_count[c];
-      dst_cnt[t] = own;
-    }
-    __syncthreads();
-  }
-  for (int64_t r = tid; r < v.count; r += 256) {
-    const int64_t j = v.slot(r);
-    int cx, cy, cz;
-    uint32_t tile, cit;
-    cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile, cit);
-    int64_t o;
-    if (own_tile && (int)tile == t) {
-      const unsigned int k = atomicAdd(&s_rank[cit], 1u);
-      unsigned int pos;
-      if (k < (unsigned)KT) {
-        pos = s_A[k] + s_T[k][cit];
-      } else {
-        pos = 0;
-        for (int c = 0; c < NC; ++c) {
-          const unsigned int n = s_count[c];
-          pos += n < k ? n : k;
-          if (c < (int)cit && n > k) ++pos;
-        }
-      }
-      o = seg[t] + pos;
-    } else {
-      const unsigned int k = atomicAdd(&dst_cnt[L.ntiles], 1u);
-      if (k >= ovf_cap) continue;  // cannot happen: the region holds every particle
-      o = seg[L.ntiles] + k;
     }
 #pragma unroll
-    for (int a = 0; a < 7; ++a) dst[a * stride + o] = src[a * stride + j];
-    dst_id[o] = src_id[j];
+    for (int u = 0; u < kTileU; ++u) {
+      const int64_t r = r0 + (int64_t)u * kTileThreads;
+      if (r < v.count) {
+        int cx, cy, cz;
+        uint32_t tile, cit;
+        cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
+        const bool home = (int)tile == t;
+        if (home) atomicAdd(&s_count[cit], 1u);
+        if (staged) s_cell[r] = home ? (unsigned char)cit : kForeign;
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = tid; c < NC; c += kTileThreads) atomicMax(&s_kmax, s_count[c]);
+  __syncthreads();
+
+  // 2. slot tables, then each particle's slot
+  const int kmax = (int)(s_kmax < (unsigned)KT ? s_kmax : (unsigned)KT);
+  for (int k = tid; k < kmax; k += kTileThreads) {
+    unsigned int a = 0, before = 0;
+    for (int c = 0; c < NC; ++c) {
+      const unsigned int n = s_count[c];
+      a += n < (unsigned)k ? n : (unsigned)k;
+      s_T[c][k] = (unsigned char)before;
+      before += n > (unsigned)k ? 1u : 0u;
+    }
+    s_A[k] = a;
+  }
+  if (tid == 0) {
+    unsigned int n_own = 0;
+    for (int c = 0; c < NC; ++c) n_own += s_count[c];
+    s_own = n_own;
+  }
+  __syncthreads();
+  const int64_t out0 = dst_seg[t];
+  for (int64_t base = 0; base < v.count; base += kTileThreads) {
+    const int64_t r = base + tid;
+    const bool in = r < v.count;
+    unsigned int c = kForeign;
+    uint32_t tile = 0;
+    if (in) {
+      if (staged) c = s_cell[r];
+      if (!staged || c == kForeign) {
+        const int64_t j = v.slot(r);
+        int cx, cy, cz;
+        uint32_t cit;
+        cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
+                   cit);
+        c = (int)tile == t ? cit : kForeign;
+      }
+    }
+    const bool foreign = in && c == kForeign;
+    const uint32_t kf = wave_rank(fill, tile, foreign);
+    if (!in) continue;
+    if (foreign) {
+      copy_particle(src, src_id, stride, v.slot(r), dst, dst_id,
+                    dst_seg[tile] + own[tile] + kf);
+      if (staged) s_slot[r] = kNoSlot;
+      continue;
+    }
+    const unsigned int k = atomicAdd(&s_rank[c], 1u);
+    unsigned int pos;
+    if (k < (unsigned)KT) {
+      pos = s_A[k] + s_T[c][k];
+    } else {
+      pos = 0;
+      for (int cc = 0; cc < NC; ++cc) {
+        const unsigned int n = s_count[cc];
+        pos += n < k ? n : k;
+        if (cc < (int)c && n > k) ++pos;
+      }
+    }
+    if (staged) s_slot[r] = (uint16_t)pos;
+    else copy_particle(src, src_id, stride, v.slot(r), dst, dst_id, out0 + pos);
+  }
+  if (!staged) return;
+  __syncthreads();  // slots complete; the tables' LDS becomes the stage
+
+  // 3. per component and output chunk: gather into the stage, write out
+  const unsigned int n_own = s_own;
+  T *stage = reinterpret_cast<T *>(s_raw);
+  uint32_t *stage_id = reinterpret_cast<uint32_t *>(s_raw);
+  for (int a = 0; a < 8; ++a) {
+    for (unsigned int c0 = 0; c0 < n_own; c0 += (unsigned)kChunk) {
+      for (int64_t r0 = tid; r0 < v.count; r0 += kTileU * kTileThreads) {
+        unsigned int p[kTileU];
+        T val[kTileU];
+        uint32_t idv[kTileU];
+#pragma unroll
+        for (int u = 0; u < kTileU; ++u) {
+          const int64_t r = r0 + (int64_t)u * kTileThreads;
+          p[u] = kChunk;
+          if (r < v.count && s_slot[r] != kNoSlot) {
+            p[u] = (unsigned int)s_slot[r] - c0;  // wraps above kChunk below c0
+            if (p[u] < (unsigned)kChunk) {
+              const int64_t j = v.slot(r);
+              if (a < 7) val[u] = src[a * stride + j];
+              else idv[u] = src_id[j];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kTileU; ++u) {
+          if (p[u] < (unsigned)kChunk) {
+            if (a < 7) stage[p[u]] = val[u];
+            else stage_id[p[u]] = idv[u];
+          }
+        }
+      }
+      __syncthreads();
+      const unsigned int m = n_own - c0 < (unsigned)kChunk ? n_own - c0 : (unsigned)kChunk;
+      for (unsigned int p = tid; p < m; p += kTileThreads) {
+        if (a < 7) dst[a * stride + out0 + c0 + p] = stage[p];
+        else dst_id[out0 + c0 + p] = stage_id[p];
+      }
+      __syncthreads();
+    }
   }
 }
 
 template <typename T>
-hipError_t tile_reorder(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
-                        const uint32_t *src_id, const int64_t *seg,
-                        const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
-                        uint32_t *dst_id, unsigned int *dst_cnt, hipStream_t s) {
+hipError_t tile_sort(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
+                     const uint32_t *src_id, const int64_t *src_seg,
+                     const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
+                     uint32_t *dst_id, int64_t *dst_seg, unsigned int *dst_cnt,
+                     void *workspace, size_t workspace_bytes, size_t *needed, hipStream_t s) {
   const int nc = L.tx * L.ty * L.tz;
-  const dim3 grid(L.ntiles + 1);
+  if (nc != 64 && nc != 128) return hipErrorInvalidValue;
+  const size_t nt1 = (size_t)L.ntiles + 1;
+  const size_t a_t = align_up(sizeof(unsigned int) * nt1);
+  size_t scan_bytes = 0;
+  hipError_t e = rocprim::exclusive_scan(nullptr, scan_bytes, (const unsigned int *)nullptr,
+                                         (unsigned int *)nullptr, 0u, nt1,
+                                         rocprim::plus<unsigned int>(), s);
+  if (e != hipSuccess) return e;
+  const size_t total = 4 * a_t + align_up(scan_bytes);
+  if (needed) *needed = total;
+  if (!workspace) return hipSuccess;
+  if (workspace_bytes < total) return hipErrorInvalidValue;
+  char *w = (char *)workspace;
+  unsigned int *count = (unsigned int *)w;          // count, own, fill: zeroed together
+  unsigned int *own = (unsigned int *)(w + a_t);
+  unsigned int *fill = (unsigned int *)(w + 2 * a_t);
+  unsigned int *P = (unsigned int *)(w + 3 * a_t);
+  void *scan_tmp = (void *)(w + 4 * a_t);
+  const int64_t nzero = (int64_t)(3 * a_t / sizeof(unsigned int));
+  hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(nzero)), dim3(256), 0, s, count, nzero);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const dim3 grid(L.ntiles + kTileOvfBlocks);
+  hipLaunchKernelGGL(k_tile_count<T>, grid, dim3(kTileThreads), 0, s, g, L, src, stride,
+                     src_seg, src_cnt, ovf_cap, count, own);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = rocprim::exclusive_scan(scan_tmp, scan_bytes, (const unsigned int *)count, P, 0u, nt1,
+                              rocprim::plus<unsigned int>(), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tile_bounds, dim3(nblocks((long)nt1)), dim3(256), 0, s,
+                     (const unsigned int *)P, (const unsigned int *)count, L.ntiles, L.slack,
+                     dst_seg, dst_cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   if (nc == 64)
-    hipLaunchKernelGGL((k_tile_reorder<T, 64, 512>), grid, dim3(256), 0, s, g, L, src, stride,
-                       src_id, seg, src_cnt, ovf_cap, dst, dst_id, dst_cnt);
-  else if (nc == 128)
-    hipLaunchKernelGGL((k_tile_reorder<T, 128, 256>), grid, dim3(256), 0, s, g, L, src,
-                       stride, src_id, seg, src_cnt, ovf_cap, dst, dst_id, dst_cnt);
+    hipLaunchKernelGGL((k_tile_sort<T, 64, 512>), grid, dim3(kTileThreads), 0, s, g, L, src,
+                       stride, src_id, src_seg, src_cnt, ovf_cap, dst, dst_id,
+                       (const int64_t *)dst_seg, (const unsigned int *)own, fill);
   else
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tile_sort<T, 128, 256>), grid, dim3(kTileThreads), 0, s, g, L, src,
+                       stride, src_id, src_seg, src_cnt, ovf_cap, dst, dst_id,
+                       (const int64_t *)dst_seg, (const unsigned int *)own, fill);
   return hipGetLastError();
 }
 
-template hipError_t tile_reorder<double>(const KGrid<double> &, const BinLayout &,
-                                         const double *, int64_t, const uint32_t *,
-                                         const int64_t *, const unsigned int *, unsigned int,
-                                         double *, uint32_t *, unsigned int *, hipStream_t);
-template hipError_t tile_reorder<float>(const KGrid<float> &, const BinLayout &,
-                                        const float *, int64_t, const uint32_t *,
-                                        const int64_t *, const unsigned int *, unsigned int,
-                                        float *, uint32_t *, unsigned int *, hipStream_t);
+template hipError_t tile_sort<double>(const KGrid<double> &, const BinLayout &, const double *,
+                                      int64_t, const uint32_t *, const int64_t *,
+                                      const unsigned int *, unsigned int, double *, uint32_t *,
+                                      int64_t *, unsigned int *, void *, size_t, size_t *,
+                                      hipStream_t);
+template hipError_t tile_sort<float>(const KGrid<float> &, const BinLayout &, const float *,
+                                     int64_t, const uint32_t *, const int64_t *,
+                                     const unsigned int *, unsigned int, float *, uint32_t *,
+                                     int64_t *, unsigned int *, void *, size_t, size_t *,
+                                     hipStream_t);
 
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;

@@ -23,7 +23,7 @@ struct spic_engine {
   spic::BinLayout L{};
   int slack_pct = 12;
   int sort_every = 0;
-  int sort_mode = 0;               // 0 full sort; 1 in-tile reorder (see opts)
+  int sort_mode = SPIC_SORT_TILE;  // re-sorts: tile_sort or bin_sort
   int lds_fp64 = 1;
   int set_moments = 0;             // phases write the moments (else add)
   int64_t phases = 0, sorts = 0;
@@ -44,7 +44,6 @@ struct spic_engine {
     int64_t *seg[2] = {nullptr, nullptr};       // [ntiles + 1] per layout
     int cur = 0;                   // buffer holding the particles
     int segi = 0;                  // layout in use
-    int64_t resorts = 0;           // sorts since load
   } sp[spic::kMaxSpecies];
   unsigned long long *counters = nullptr;  // see BinnedArgs::counters
   void *partials = nullptr;        // see BinnedArgs::partials
@@ -97,32 +96,37 @@ hipError_t ensure_workspace(spic_engine *e, size_t need, hipStream_t s) {
   return hipSuccess;
 }
 
-// In-tile reorder of species `is` into the other buffer, same layout.
+// Sort species `is` into the other buffer and the other layout.  With
+// `src` given, the particles come densely from the caller's arrays instead.
+// A re-sort in SPIC_SORT_TILE mode is a tile sort.
 template <typename T>
-hipError_t reorder_species(spic_engine *e, int is, hipStream_t s) {
+hipError_t tile_sort_species(spic_engine *e, int is, hipStream_t s) {
   auto &sp = e->sp[is];
-  const int from = sp.cur, to = 1 - sp.cur;
-  const size_t ncnt = 3 * (size_t)(e->L.ntiles + 1);
-  hipError_t err = spic::zero_u32(sp.cnt[to], ncnt, s);
+  spic::BinLayout L = e->L;
+  L.slack = sp.slack;
+  const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
+  const int from = sp.cur, to = 1 - sp.cur, lay_to = 1 - sp.segi;
+  size_t need = 0;
+  hipError_t err = spic::tile_sort<T>(g, L, nullptr, sp.cap, nullptr, nullptr, nullptr,
+                                      sp.ovf_cap, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, 0, &need, s);
   if (err != hipSuccess) return err;
-  err = spic::tile_reorder<T>(spic::kgrid<T>(&e->grd), e->L, (const T *)sp.block[from],
-                              sp.cap, sp.id[from], sp.seg[sp.segi], sp.cnt[from],
-                              sp.ovf_cap, (T *)sp.block[to], sp.id[to], sp.cnt[to], s);
+  if ((err = ensure_workspace(e, need, s)) != hipSuccess) return err;
+  err = spic::tile_sort<T>(g, L, (const T *)sp.block[from], sp.cap, sp.id[from],
+                           sp.seg[sp.segi], sp.cnt[from], sp.ovf_cap, (T *)sp.block[to],
+                           sp.id[to], sp.seg[lay_to], sp.cnt[to], e->ws, e->ws_bytes,
+                           nullptr, s);
   if (err != hipSuccess) return err;
   sp.cur = to;
+  sp.segi = lay_to;
   e->sorts++;
   return hipSuccess;
 }
 
-// Sort species `is` into the other buffer and the other layout.  With
-// `src` given, the particles come densely from the caller's arrays instead.
-// In sort mode 1 a re-sort is an in-tile reorder, except every fourth,
-// which re-sizes the segments to the current counts.
 template <typename T>
 hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
   auto &sp = e->sp[is];
-  if (src) sp.resorts = 0;
-  else if (e->sort_mode == 1 && (sp.resorts++ % 4) != 3) return reorder_species<T>(e, is, s);
+  if (!src && e->sort_mode == SPIC_SORT_TILE) return tile_sort_species<T>(e, is, s);
   const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
   spic::BinLayout L = e->L;
   L.slack = sp.slack;
@@ -260,7 +264,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
   int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0,
-      sort_mode = 0;
+      sort_mode = SPIC_SORT_TILE;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -275,8 +279,8 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->set_moments != 0 && opts->set_moments != 1)
       return fail(SPIC_EINVAL, "set_moments must be 0 or 1");
     set_mom = opts->set_moments;
-    if (opts->sort_mode != 0 && opts->sort_mode != 1)
-      return fail(SPIC_EINVAL, "sort_mode must be 0 or 1");
+    if (opts->sort_mode != SPIC_SORT_TILE && opts->sort_mode != SPIC_SORT_RADIX)
+      return fail(SPIC_EINVAL, "sort_mode must be SPIC_SORT_TILE or SPIC_SORT_RADIX");
     sort_mode = opts->sort_mode;
   }
   // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
@@ -725,9 +729,8 @@ int spic_batch_phase(spic_batch *b, void *stream) {
     if (err != hipSuccess) return hip_fail(err, "batch argument tables");
   }
   if (sort_now) {
-    // A full sort moves to the other buffer and the other layout, an
-    // in-tile reorder to the other buffer only; members sorted in step
-    // land in one state.
+    // A sort moves each species to the other buffer and the other layout;
+    // members in step stay in one state.
     int after = -1;
     for (int i = 0; i < b->n; ++i) {
       spic_engine *e = b->e[i];

@@ -112,12 +112,11 @@ def test_batch_matches_oracle_per_instance(prec, tile, sort_every):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_batch_in_tile_reorder_matches_oracle(prec):
-    """Members in sort mode 1, re-sorted every phase for six phases: the
-    batch follows the members' buffer and layout through in-tile reorders
-    (buffer only) and the fourth re-sort (buffer and layout)."""
+def test_batch_radix_sort_matches_oracle(prec):
+    """Members whose re-sorts are radix sorts, re-sorted every phase for
+    six phases (the other batch tests re-sort by tile sort, the default)."""
     states = [_oracle(spec, prec) for spec in INSTANCES]
-    insts = [Instance(st, prec, sort_mode=1) for st in states]
+    insts = [Instance(st, prec, sort_mode=S.SPIC_SORT_RADIX) for st in states]
     batch = S.Batch([x.member() for x in insts], sort_every=1, set_moments=True)
     for _ in range(6):
         batch.phase()
@@ -228,7 +227,7 @@ def test_batch_rejects_bad_members():
     with pytest.raises(S.SpicError) as ei:
         S.Batch([a.member(), c.member()])
     assert ei.value.code == S.SPIC_EINVAL
-    e = Instance(_oracle(INSTANCES[1], "f64"), "f64", sort_mode=1)
+    e = Instance(_oracle(INSTANCES[1], "f64"), "f64", sort_mode=S.SPIC_SORT_RADIX)
     with pytest.raises(S.SpicError) as ei:
         S.Batch([a.member(), e.member()])
     assert ei.value.code == S.SPIC_EINVAL

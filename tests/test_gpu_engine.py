@@ -125,14 +125,14 @@ def test_engine_tile_shapes_and_sorting_do_not_change_trajectories(tile, sort_ev
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("tile", [(4, 4, 4), (8, 4, 4)])
-def test_engine_in_tile_reorder_keeps_trajectories(prec, tile):
-    """sort_mode 1, a re-sort every phase over seven phases: re-sorts 1-3
-    and 5-6 are in-tile reorders, the fourth a full sort.  Particles, ids
-    and counters pass through both kinds bit-exactly."""
+@pytest.mark.parametrize("sort_mode", [S.SPIC_SORT_TILE, S.SPIC_SORT_RADIX])
+def test_engine_sort_modes_keep_trajectories(prec, tile, sort_mode):
+    """A re-sort every phase over seven phases, tile or radix sorts:
+    particles, ids and counters pass through every re-sort bit-exactly."""
     st = oracle_small(prec)
     S.set_strict_fp(True)
     got_p, got_m, stats = run_engine(st, prec, phases=7, tile=tile, sort_every=1,
-                                     sort_mode=1)
+                                     sort_mode=sort_mode)
     oracle_phases(st, 7)
     assert stats["sorts"] == st.cfg.ns * 7
     for s in range(st.cfg.ns):
@@ -143,19 +143,51 @@ def test_engine_in_tile_reorder_keeps_trajectories(prec, tile):
                          f"s{s} {n}")
 
 
-def test_engine_in_tile_reorder_moves_foreign_particles_to_overflow():
-    """gem_hot without E (speeds below a cell per step) re-sorted every
-    phase in sort mode 1: particles that left their tile since the last
-    full sort go to the overflow region, and the next phase still finds
-    and moves them exactly."""
+@pytest.mark.parametrize("sort_mode", [S.SPIC_SORT_TILE, S.SPIC_SORT_RADIX])
+def test_engine_sort_places_crossing_particles_in_their_tiles(sort_mode):
+    """gem_hot without fields, at rest, except species 0's last cell layer
+    of tile 0, which drifts +0.9 cell per step into tile 1: more particles
+    than tile 1's segment has slack for, so the first phase puts some into
+    the overflow region.  A sort, in either mode, moves every particle into
+    its tile's segment (an empty overflow region) and the trajectories stay
+    exact."""
     data = load_case("gem_hot", "f64")
     st = oracle_from_fixture("gem_hot", "f64", data)
-    for n in ("Ex", "Ey", "Ez"):
+    for n in O.FIELD_NAMES:
         st.fields[n][:] = 0.0
+    c = st.cfg
+    dx = c.Lx / c.nxc
+    for s_, p in enumerate(st.parts):
+        for k in ("u", "v", "w"):
+            p[k][:] = 0.0
+        if s_ == 0:
+            last = (p["x"] >= 3 * dx) & (p["x"] < 4 * dx)
+            p["u"][last] = 0.9 * dx / c.dt
     S.set_strict_fp(True)
-    got_p, got_m, stats = run_engine(st, "f64", phases=6, sort_every=1, sort_mode=1)
-    oracle_phases(st, 6)
-    for s in range(st.cfg.ns):
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f64")
+    field = S.EMField(grd, {n: st.fields[n] for n in O.FIELD_NAMES})
+    parts = [S.Particles(p, c.qom[s], "f64", c.NiterMover) for s, p in enumerate(st.parts)]
+    ids = [S.InterpDensSpecies(grd) for _ in range(c.ns)]
+    param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
+                         PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
+    eng = S.Engine(grd, c.ns, sort_mode=sort_mode)
+    for s, p in enumerate(parts):
+        eng.load(s, p)
+    spilled = 0
+    for _ in range(4):
+        S.setZeroSpeciesDensities(ids, grd, c.ns)
+        eng.phase(field, ids, param)
+        spilled = max(spilled, eng.stats()["overflow"])
+        eng.sort()
+        assert eng.stats()["overflow"] == 0
+    assert spilled > 0
+    for s, p in enumerate(parts):
+        eng.store(s, p)
+    got_p = [p.download() for p in parts]
+    got_m = [m.download() for m in ids]
+    eng.close()
+    oracle_phases(st, 4)
+    for s in range(c.ns):
         for k in O.PARTICLE_NAMES:
             assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
         for n in O.MOMENT_NAMES:
