@@ -205,6 +205,8 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         "particles": n_total,
         "kernel": kernel,
         "avg_kernel_ms": avg_kernel_ms,
+        "median_kernel_ms": sorted(kernel_ms)[len(kernel_ms) // 2],
+        "kernel_ms_range": [min(kernel_ms), max(kernel_ms)],
         "bytes_per_launch": per_launch_bytes,
         "achieved_GBs": achieved,
         "roofline_frac": achieved / HBM_PEAK_GBS,
@@ -424,7 +426,7 @@ def run_batch(steps: int, warmup: int) -> dict:
     stream = S.Stream()
     t0 = time.perf_counter()
     tuned = tune_batch(lambda tile: [GemInstance(sp, tile, stream) for sp in specs],
-                       phases=6, warmup=1, stream=stream, set_moments=True)
+                       phases=10, warmup=1, stream=stream, set_moments=True)
     t_tune = time.perf_counter() - t0
     log(f"batch: tuned in {t_tune:.1f} s: " + ", ".join(
         f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}: {t.ms_per_phase:.3f} ms"
@@ -740,6 +742,54 @@ class _StdoutToStderr:
         return False
 
 
+def copy_bandwidth(nbytes: int = 4 << 30, reps: int = 5) -> dict:
+    """A plain device-to-device copy of nbytes (hipMemcpy), timed with HIP
+    events on the null stream; GB/s counts the bytes read plus the bytes
+    written.  The median of `reps` copies, after one untimed copy."""
+    import sputnipic_amd as S
+    from sputnipic_amd._capi import check, lib
+
+    n = nbytes // 4
+    a = S.DeviceArray(n, "float32", fill_zero=True)
+    b = S.DeviceArray(n, "float32")
+    times = []
+    try:
+        for r in range(reps + 1):
+            e0, e1 = S.Event(), S.Event()
+            e0.record(None)
+            check(lib().spic_memcpy_d2d(b.ptr, a.ptr, n * 4), "copy")
+            e1.record(None)
+            check(lib().spic_synchronize(), "synchronize")
+            if r:
+                times.append(e0.elapsed_ms(e1))
+    finally:
+        a.free()
+        b.free()
+    ms = sorted(times)[len(times) // 2]
+    gbs = 2 * n * 4 / (ms * 1e-3) / 1e9
+    log(f"device copy: {n * 4} bytes in {ms:.3f} ms (median of {reps}): {gbs:.0f} GB/s "
+        "read + write")
+    return {"bytes": n * 4, "ms": ms, "GBs": gbs, "reps": reps}
+
+
+def host_cpu() -> dict:
+    """The host's CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(cycles: int) -> dict:
     """The reference's own CPU mover_PC + interpP2G (its source compiled at
     the optimisation level of its Makefile, oracle/_ref/libref_f64_O3.so),
@@ -776,6 +826,7 @@ def cpu_baseline(cycles: int) -> dict:
                   f"particles, FP64, {cycles} cycles of mover_PC + zero + interpP2G "
                   f"(OpenMP over particles in the mover, over species in the deposit)",
         "seconds": total,
+        "host": host_cpu(),
     }
 
 
@@ -954,6 +1005,8 @@ def main() -> int:
     if not args.no_field:
         with _StdoutToStderr():
             field = run_field(20, cpu=not args.no_cpu_baseline)
+    with _StdoutToStderr():
+        copy = copy_bandwidth()
     line = {
         "metric": METRIC,
         "value": f["value"],
@@ -986,7 +1039,10 @@ def main() -> int:
             "kernel": f["kernel"],
             "avg_kernel_ms": f["avg_kernel_ms"],
             "algorithmic_bytes_per_launch": f["bytes_per_launch"],
+            "device_copy_GBs": copy["GBs"],
+            "frac_of_device_copy": f["achieved_GBs"] / copy["GBs"],
         },
+        "device_copy": copy,
         "forms": {k: {kk: vv for kk, vv in v.items()} for k, v in forms.items()},
         "cpu_baseline": cpu,
         "field_solve": field,

@@ -219,21 +219,55 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, T *const dst[7], hipStream_t s);
 
-// Tile sort of one binned species from buffer src (layout src_seg) to
-// buffer dst (layout dst_seg): the binned source's alternative to bin_sort,
-// by counting within tiles instead of a radix sort over all particles.
-// The destination gets bin_sort's segments (count + slack slots each, an
-// empty overflow region); each tile holds its own particles (those that
-// were in its segment and are in its cells) in round-robin cell order --
-// bin_sort's order -- followed by the particles that came from other
-// segments or the overflow region.  Component c of slot i at
-// base[c * stride + i].  With workspace == nullptr only *needed is set.
+// Tile sort of one binned species (a job) from buffer src (layout
+// src_seg) to buffer dst (layout dst_seg): the binned source's alternative
+// to bin_sort, by counting within tiles instead of a radix sort over all
+// particles.  The destination gets bin_sort's segments (count + L.slack
+// slots each, an empty overflow region); each tile holds its own particles
+// (those that were in its segment and are in its cells) in round-robin
+// cell order -- bin_sort's order -- followed by the particles that came
+// from other segments or the overflow region.  Component c of slot i at
+// base[c * stride + i].  count, own and fill: workspace of ntiles + 1 words
+// each, zeroed by the launchers.
 template <typename T>
-hipError_t tile_sort(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
-                     const uint32_t *src_id, const int64_t *src_seg,
-                     const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
-                     uint32_t *dst_id, int64_t *dst_seg, unsigned int *dst_cnt,
-                     void *workspace, size_t workspace_bytes, size_t *needed, hipStream_t s);
+struct TileSortJob {
+  KGrid<T> g;
+  BinLayout L;
+  const T *src;
+  T *dst;
+  int64_t stride;
+  const uint32_t *src_id;
+  uint32_t *dst_id;
+  const int64_t *src_seg;
+  int64_t *dst_seg;
+  const unsigned int *src_cnt;
+  unsigned int *dst_cnt;
+  unsigned int ovf_cap;
+  unsigned int *count, *own, *fill;
+};
+
+// Workgroups of the count and sort launches per job: one per tile, plus
+// kTileOvfBlocks over the overflow region.
+constexpr int kTileOvfBlocks = 64;
+
+// Jobs of a batched tile sort: job j owns blocks start[j] .. start[j+1]-1
+// of the count and sort launches.
+struct TileSortMap {
+  int start[kMaxInstances * kMaxSpecies + 1];
+  int n;
+};
+
+// One job; count, own and fill must be one block of 3 * (ntiles + 1)
+// words starting at count.
+template <typename T>
+hipError_t tile_sort(const TileSortJob<T> &job, hipStream_t s);
+
+// map.n jobs from the device table `jobs`, all of one tile shape, whose
+// count / own / fill arrays all lie in ws[0 .. ws_words).
+template <typename T>
+hipError_t tile_sort_batch(const TileSortJob<T> *jobs, const TileSortMap &map, int tx,
+                           int ty, int tz, unsigned int *ws, size_t ws_words,
+                           hipStream_t s);
 
 // bin_store of x y z u v w only, as big-endian IEEE float32 words (values
 // rounded to float): the particle datasets of a snapshot.

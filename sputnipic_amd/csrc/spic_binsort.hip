@@ -399,38 +399,80 @@ template hipError_t launch_batch_zero_counts<float>(const BinnedArgs<float> *,
                                                     const BatchMap &, int, int, hipStream_t);
 
 // Copy particle j of the source to slot o of the destination.
-template <typename T>
-__device__ __forceinline__ void copy_particle(const T *__restrict__ src,
-                                              const uint32_t *__restrict__ src_id,
-                                              int64_t stride, int64_t j, T *__restrict__ dst,
-                                              uint32_t *__restrict__ dst_id, int64_t o) {
+template <typename PS, typename PSI, typename PD, typename PDI>
+__device__ __forceinline__ void copy_particle(PS src, PSI src_id, int64_t stride, int64_t j,
+                                              PD dst, PDI dst_id, int64_t o) {
 #pragma unroll
   for (int a = 0; a < 7; ++a) dst[a * stride + o] = src[a * stride + j];
   dst_id[o] = src_id[j];
 }
 
 constexpr int kTileThreads = 512;
-constexpr int kTileOvfBlocks = 64;          // blocks that split the overflow region
 constexpr unsigned int kTileStaged = 12288;  // particles a tile block stages in LDS
 constexpr int kTileStageBytes = 32768;       // one output chunk of one component
 constexpr int kTileU = 4;                    // particles a thread loads at once
+constexpr int kScanThreads = 1024;
 
-// Tile sort, step 1.  Block t < ntiles reads segment t: its particles that
-// lie in tile t are tile t's own (own[t], added to count[t]); every other
-// particle adds one to its tile's count.  Blocks ntiles and up split the
-// overflow region, whose particles all add one to their tile's count.
-template <typename T>
+// wave_rank on a global counter array (global atomics, not flat ones).
+__device__ __forceinline__ uint32_t wave_rank_g(SPIC_GLOBAL unsigned int *counter,
+                                                uint32_t key, bool active) {
+  const int lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(active);
+  uint32_t rank = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(active && key == lkey);
+    unsigned int base = 0;
+    if (lane == leader)
+      base = __hip_atomic_fetch_add(counter + lkey, (unsigned int)__popcll(same),
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base = __shfl(base, leader);
+    if (active && key == lkey) rank = base + (uint32_t)__popcll(same & below);
+    todo &= ~same;
+  }
+  return rank;
+}
+
+// Job of block b in a tile-sort map (uniform across the workgroup).
+__device__ __forceinline__ int sort_job(const TileSortMap &map, int b) {
+  int j = 0;
+  for (int k = 1; k < map.n; ++k) j += b >= map.start[k] ? 1 : 0;
+  return j;
+}
+
+// Tile sort, step 1 (block b of nb of the job).  Block t < ntiles reads
+// segment t: its particles that lie in tile t are tile t's own (own[t],
+// added to count[t]); every other particle adds one to its tile's count.
+// Blocks ntiles and up split the overflow region, whose particles all add
+// one to their tile's count.
+// BATCH: the launch covers every job of a map; the block's job, taken from
+// the table, replaces the pointer and layout arguments.
+template <typename T, bool BATCH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_count(
-    KGrid<T> g, BinLayout L, const T *__restrict__ src, int64_t stride,
-    const int64_t *__restrict__ seg, const unsigned int *__restrict__ cnt,
-    unsigned int ovf_cap, unsigned int *__restrict__ count, unsigned int *__restrict__ own) {
+    KGrid<T> g, BinLayout L, const T *src_a, int64_t stride, const int64_t *seg_a,
+    const unsigned int *cnt_a, unsigned int ovf_cap, unsigned int *count_a,
+    unsigned int *own_a, const TileSortJob<T> *__restrict__ jobs, TileSortMap map) {
   __shared__ unsigned int s_own;
-  const int t = blockIdx.x, tid = threadIdx.x;
-  const bool ovf = t >= L.ntiles;
-  const RangeView v = range_view(seg, cnt, L.ntiles, ovf_cap, ovf ? L.ntiles : t);
-  const int64_t first = ovf ? (int64_t)(t - L.ntiles) * kTileThreads : 0;
-  const int64_t step =
-      ovf ? (int64_t)(gridDim.x - L.ntiles) * kTileThreads : (int64_t)kTileThreads;
+  int b = blockIdx.x;
+  int nb = gridDim.x;
+  if constexpr (BATCH) {
+    const int j = sort_job(map, blockIdx.x);
+    const TileSortJob<T> &J = jobs[j];
+    g = J.g; L = J.L; src_a = J.src; stride = J.stride; seg_a = J.src_seg;
+    cnt_a = J.src_cnt; ovf_cap = J.ovf_cap; count_a = J.count; own_a = J.own;
+    b -= map.start[j];
+    nb = map.start[j + 1] - map.start[j];
+  }
+  const SPIC_GLOBAL T *__restrict__ src = gmem(src_a);
+  SPIC_GLOBAL unsigned int *__restrict__ count = gmem(count_a);
+  const int tid = threadIdx.x;
+  const bool ovf = b >= L.ntiles;
+  const RangeView v = range_view(gmem(seg_a), gmem(cnt_a), L.ntiles, ovf_cap,
+                                 ovf ? L.ntiles : b);
+  const int64_t first = ovf ? (int64_t)(b - L.ntiles) * kTileThreads : 0;
+  const int64_t step = ovf ? (int64_t)(nb - L.ntiles) * kTileThreads : (int64_t)kTileThreads;
   if (tid == 0) s_own = 0u;
   __syncthreads();
   unsigned int mine = 0;
@@ -455,42 +497,76 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_count(
         uint32_t cit;
         cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
       }
-      const bool home = in && !ovf && (int)tile == t;
+      const bool home = in && !ovf && (int)tile == b;
       mine += home ? 1u : 0u;
-      (void)wave_rank(count, tile, in && !home);
+      (void)wave_rank_g(count, tile, in && !home);
     }
   }
   if (ovf) return;
   if (mine) atomicAdd(&s_own, mine);
   __syncthreads();
   if (tid == 0) {
-    own[t] = s_own;
-    if (s_own) atomicAdd(&count[t], s_own);
+    gmem(own_a)[b] = s_own;
+    if (s_own)
+      __hip_atomic_fetch_add(count + b, s_own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Tile sort, step 2: the destination layout from the exclusive scan P of
-// the counts (P[ntiles] = n): seg[t] = P[t] + t * slack, cnt[t] = count[t],
-// no unused runs, an empty overflow region.
-__global__ void k_tile_bounds(const unsigned int *__restrict__ P,
-                              const unsigned int *__restrict__ count, int ntiles, int64_t slack,
-                              int64_t *__restrict__ seg, unsigned int *__restrict__ cnt) {
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
-       t += gridDim.x * blockDim.x) {
-    seg[t] = (int64_t)P[t] + (int64_t)t * slack;
-    cnt[t] = t < ntiles ? count[t] : 0u;
-    cnt[(ntiles + 1) + t] = 0u;
-    cnt[2 * (ntiles + 1) + t] = 0u;
+// Tile sort, step 2 (one workgroup of kScanThreads): the exclusive scan P
+// of the counts (P[ntiles] = n) and the destination layout,
+// seg[t] = P[t] + t * slack, cnt[t] = count[t], no unused runs, an empty
+// overflow region.
+template <typename T>
+__device__ __forceinline__ void tile_bounds_body(const TileSortJob<T> &J) {
+  constexpr int NW = kScanThreads / 64;
+  __shared__ unsigned int s_wsum[NW];
+  __shared__ unsigned int s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nt = J.L.ntiles;
+  if (tid == 0) s_carry = 0u;
+  __syncthreads();
+  for (int base = 0; base <= nt; base += kScanThreads) {
+    const int t = base + tid;
+    const unsigned int c = t < nt ? gmem(J.count)[t] : 0u;
+    unsigned int v = c;  // inclusive scan within the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned int u = __shfl_up(v, d);
+      if (lane >= d) v += u;
+    }
+    if (lane == 63) s_wsum[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+      unsigned int w = lane < NW ? s_wsum[lane] : 0u;
+#pragma unroll
+      for (int d = 1; d < NW; d <<= 1) {
+        const unsigned int u = __shfl_up(w, d);
+        if (lane >= d) w += u;
+      }
+      if (lane < NW) s_wsum[lane] = w;  // inclusive over the waves
+    }
+    __syncthreads();
+    const unsigned int carry = s_carry;
+    const unsigned int excl = carry + (wid ? s_wsum[wid - 1] : 0u) + v - c;
+    if (t <= nt) {
+      gmem(J.dst_seg)[t] = (int64_t)excl + (int64_t)t * J.L.slack;
+      gmem(J.dst_cnt)[t] = c;
+      gmem(J.dst_cnt)[(nt + 1) + t] = 0u;
+      gmem(J.dst_cnt)[2 * (nt + 1) + t] = 0u;
+    }
+    __syncthreads();
+    if (tid == 0) s_carry = carry + s_wsum[NW - 1];
+    __syncthreads();
   }
 }
 
-// Tile sort, step 3.  Block t < ntiles writes tile t's own particles to
-// the start of its destination segment in round-robin cell order -- rank 0
-// of every cell in cell order, then rank 1, and so on: bin_sort's order --
-// and every other particle of its source segment after the own particles
-// of that particle's tile (slot dst_seg[tile] + own[tile] + k, k from the
-// fill counters).  Blocks ntiles and up do the latter for the overflow
-// region.  For the own particles:
+// Tile sort, step 3 (block b of nb of the job).  Block t < ntiles writes
+// tile t's own particles to the start of its destination segment in
+// round-robin cell order -- rank 0 of every cell in cell order, then rank
+// 1, and so on: bin_sort's order -- and every other particle of its source
+// segment after the own particles of that particle's tile (slot
+// dst_seg[tile] + own[tile] + k, k from the fill counters).  Blocks ntiles
+// and up do the latter for the overflow region.  For the own particles:
 //  1. each particle's cell within the tile (or "foreign") goes to LDS, and
 //     LDS counters count each cell;
 //  2. tables give the slot of the k-th particle of cell c as A[k] + T[c][k]
@@ -503,13 +579,14 @@ __global__ void k_tile_bounds(const unsigned int *__restrict__ P,
 //     slots, and the stage is written out in slot order (coalesced).
 // A segment of more than kTileStaged particles skips the stage and writes
 // each particle at its slot directly.
-template <typename T, int NC, int KT>
+// BATCH: the launch covers every job of a map; the block's job, taken from
+// the table, replaces the pointer and layout arguments.
+template <typename T, int NC, int KT, bool BATCH>
 __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
-    KGrid<T> g, BinLayout L, const T *__restrict__ src, int64_t stride,
-    const uint32_t *__restrict__ src_id, const int64_t *__restrict__ src_seg,
-    const unsigned int *__restrict__ src_cnt, unsigned int ovf_cap, T *__restrict__ dst,
-    uint32_t *__restrict__ dst_id, const int64_t *__restrict__ dst_seg,
-    const unsigned int *__restrict__ own, unsigned int *__restrict__ fill) {
+    KGrid<T> g, BinLayout L, const T *src_a, int64_t stride, const uint32_t *src_id_a,
+    const int64_t *src_seg_a, const unsigned int *src_cnt_a, unsigned int ovf_cap, T *dst_a,
+    uint32_t *dst_id_a, const int64_t *dst_seg_a, const unsigned int *own_a,
+    unsigned int *fill_a, const TileSortJob<T> *__restrict__ jobs, TileSortMap map) {
   static_assert(KT * NC <= kTileStageBytes, "slot table exceeds the stage");
   static_assert(NC <= 255, "cell index must fit a byte below the foreign mark");
   constexpr unsigned char kForeign = 0xFF;
@@ -521,12 +598,31 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
   __shared__ unsigned char s_cell[kTileStaged];
   __shared__ __align__(16) unsigned char s_raw[kTileStageBytes];
   unsigned char(*s_T)[KT] = reinterpret_cast<unsigned char(*)[KT]>(s_raw);
-  const int t = blockIdx.x;
+  int t = blockIdx.x;
+  int nb = gridDim.x;
+  if constexpr (BATCH) {
+    const int j = sort_job(map, blockIdx.x);
+    const TileSortJob<T> &J = jobs[j];
+    g = J.g; L = J.L; src_a = J.src; stride = J.stride; src_id_a = J.src_id;
+    src_seg_a = J.src_seg; src_cnt_a = J.src_cnt; ovf_cap = J.ovf_cap; dst_a = J.dst;
+    dst_id_a = J.dst_id; dst_seg_a = J.dst_seg; own_a = J.own; fill_a = J.fill;
+    t -= map.start[j];
+    nb = map.start[j + 1] - map.start[j];
+  }
+  const SPIC_GLOBAL T *__restrict__ src = gmem(src_a);
+  const SPIC_GLOBAL uint32_t *__restrict__ src_id = gmem(src_id_a);
+  const SPIC_GLOBAL int64_t *__restrict__ src_seg = gmem(src_seg_a);
+  const SPIC_GLOBAL unsigned int *__restrict__ src_cnt = gmem(src_cnt_a);
+  SPIC_GLOBAL T *__restrict__ dst = gmem(dst_a);
+  SPIC_GLOBAL uint32_t *__restrict__ dst_id = gmem(dst_id_a);
+  const SPIC_GLOBAL int64_t *__restrict__ dst_seg = gmem(dst_seg_a);
+  const SPIC_GLOBAL unsigned int *__restrict__ own = gmem(own_a);
+  SPIC_GLOBAL unsigned int *__restrict__ fill = gmem(fill_a);
   const int tid = threadIdx.x;
 
   if (t >= L.ntiles) {
     const RangeView v = range_view(src_seg, src_cnt, L.ntiles, ovf_cap, L.ntiles);
-    const int64_t step = (int64_t)(gridDim.x - L.ntiles) * kTileThreads;
+    const int64_t step = (int64_t)(nb - L.ntiles) * kTileThreads;
     for (int64_t base = (int64_t)(t - L.ntiles) * kTileThreads; base < v.count;
          base += step) {
       const int64_t r = base + tid;
@@ -540,7 +636,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
         cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
                    cit);
       }
-      const uint32_t k = wave_rank(fill, tile, in);
+      const uint32_t k = wave_rank_g(fill, tile, in);
       if (in) copy_particle(src, src_id, stride, j, dst, dst_id, dst_seg[tile] + own[tile] + k);
     }
     return;
@@ -622,7 +718,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
       }
     }
     const bool foreign = in && c == kForeign;
-    const uint32_t kf = wave_rank(fill, tile, foreign);
+    const uint32_t kf = wave_rank_g(fill, tile, foreign);
     if (!in) continue;
     if (foreign) {
       copy_particle(src, src_id, stride, v.slot(r), dst, dst_id,
@@ -691,65 +787,90 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
 }
 
 template <typename T>
-hipError_t tile_sort(const KGrid<T> &g, const BinLayout &L, const T *src, int64_t stride,
-                     const uint32_t *src_id, const int64_t *src_seg,
-                     const unsigned int *src_cnt, unsigned int ovf_cap, T *dst,
-                     uint32_t *dst_id, int64_t *dst_seg, unsigned int *dst_cnt,
-                     void *workspace, size_t workspace_bytes, size_t *needed, hipStream_t s) {
-  const int nc = L.tx * L.ty * L.tz;
+__global__ __launch_bounds__(kScanThreads) void k_tile_bounds(TileSortJob<T> J) {
+  tile_bounds_body<T>(J);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_tile_bounds_batch(
+    const TileSortJob<T> *__restrict__ jobs) {
+  const TileSortJob<T> &J = jobs[blockIdx.x];
+  tile_bounds_body<T>(J);
+}
+
+template <typename T>
+hipError_t tile_sort(const TileSortJob<T> &J, hipStream_t s) {
+  const int nc = J.L.tx * J.L.ty * J.L.tz;
   if (nc != 64 && nc != 128) return hipErrorInvalidValue;
-  const size_t nt1 = (size_t)L.ntiles + 1;
-  const size_t a_t = align_up(sizeof(unsigned int) * nt1);
-  size_t scan_bytes = 0;
-  hipError_t e = rocprim::exclusive_scan(nullptr, scan_bytes, (const unsigned int *)nullptr,
-                                         (unsigned int *)nullptr, 0u, nt1,
-                                         rocprim::plus<unsigned int>(), s);
-  if (e != hipSuccess) return e;
-  const size_t total = 4 * a_t + align_up(scan_bytes);
-  if (needed) *needed = total;
-  if (!workspace) return hipSuccess;
-  if (workspace_bytes < total) return hipErrorInvalidValue;
-  char *w = (char *)workspace;
-  unsigned int *count = (unsigned int *)w;          // count, own, fill: zeroed together
-  unsigned int *own = (unsigned int *)(w + a_t);
-  unsigned int *fill = (unsigned int *)(w + 2 * a_t);
-  unsigned int *P = (unsigned int *)(w + 3 * a_t);
-  void *scan_tmp = (void *)(w + 4 * a_t);
-  const int64_t nzero = (int64_t)(3 * a_t / sizeof(unsigned int));
-  hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(nzero)), dim3(256), 0, s, count, nzero);
+  const int64_t nzero = 3 * ((int64_t)J.L.ntiles + 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(nzero)), dim3(256), 0, s, J.count, nzero);
+  hipError_t e;
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const dim3 grid(L.ntiles + kTileOvfBlocks);
-  hipLaunchKernelGGL(k_tile_count<T>, grid, dim3(kTileThreads), 0, s, g, L, src, stride,
-                     src_seg, src_cnt, ovf_cap, count, own);
+  const dim3 grid(J.L.ntiles + kTileOvfBlocks);
+  hipLaunchKernelGGL((k_tile_count<T, false>), grid, dim3(kTileThreads), 0, s, J.g, J.L, J.src,
+                     J.stride, J.src_seg, J.src_cnt, J.ovf_cap, J.count, J.own,
+                     (const TileSortJob<T> *)nullptr, TileSortMap{});
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = rocprim::exclusive_scan(scan_tmp, scan_bytes, (const unsigned int *)count, P, 0u, nt1,
-                              rocprim::plus<unsigned int>(), s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tile_bounds, dim3(nblocks((long)nt1)), dim3(256), 0, s,
-                     (const unsigned int *)P, (const unsigned int *)count, L.ntiles, L.slack,
-                     dst_seg, dst_cnt);
+  hipLaunchKernelGGL(k_tile_bounds<T>, dim3(1), dim3(kScanThreads), 0, s, J);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (nc == 64)
-    hipLaunchKernelGGL((k_tile_sort<T, 64, 512>), grid, dim3(kTileThreads), 0, s, g, L, src,
-                       stride, src_id, src_seg, src_cnt, ovf_cap, dst, dst_id,
-                       (const int64_t *)dst_seg, (const unsigned int *)own, fill);
+    hipLaunchKernelGGL((k_tile_sort<T, 64, 512, false>), grid, dim3(kTileThreads), 0, s, J.g, J.L,
+                       J.src, J.stride, J.src_id, J.src_seg, J.src_cnt, J.ovf_cap, J.dst,
+                       J.dst_id, (const int64_t *)J.dst_seg, (const unsigned int *)J.own,
+                       J.fill, (const TileSortJob<T> *)nullptr, TileSortMap{});
   else
-    hipLaunchKernelGGL((k_tile_sort<T, 128, 256>), grid, dim3(kTileThreads), 0, s, g, L, src,
-                       stride, src_id, src_seg, src_cnt, ovf_cap, dst, dst_id,
-                       (const int64_t *)dst_seg, (const unsigned int *)own, fill);
+    hipLaunchKernelGGL((k_tile_sort<T, 128, 256, false>), grid, dim3(kTileThreads), 0, s, J.g, J.L,
+                       J.src, J.stride, J.src_id, J.src_seg, J.src_cnt, J.ovf_cap, J.dst,
+                       J.dst_id, (const int64_t *)J.dst_seg, (const unsigned int *)J.own,
+                       J.fill, (const TileSortJob<T> *)nullptr, TileSortMap{});
   return hipGetLastError();
 }
 
-template hipError_t tile_sort<double>(const KGrid<double> &, const BinLayout &, const double *,
-                                      int64_t, const uint32_t *, const int64_t *,
-                                      const unsigned int *, unsigned int, double *, uint32_t *,
-                                      int64_t *, unsigned int *, void *, size_t, size_t *,
-                                      hipStream_t);
-template hipError_t tile_sort<float>(const KGrid<float> &, const BinLayout &, const float *,
-                                     int64_t, const uint32_t *, const int64_t *,
-                                     const unsigned int *, unsigned int, float *, uint32_t *,
-                                     int64_t *, unsigned int *, void *, size_t, size_t *,
-                                     hipStream_t);
+template <typename T>
+hipError_t tile_sort_batch(const TileSortJob<T> *jobs, const TileSortMap &map, int tx,
+                           int ty, int tz, unsigned int *ws, size_t ws_words,
+                           hipStream_t s) {
+  const int nc = tx * ty * tz;
+  if (nc != 64 && nc != 128) return hipErrorInvalidValue;
+  if (map.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3(nblocks((long)ws_words)), dim3(256), 0, s, ws,
+                     (int64_t)ws_words);
+  hipError_t e;
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const dim3 grid(map.start[map.n]);
+  const KGrid<T> g0{};
+  const BinLayout L0{};
+  hipLaunchKernelGGL((k_tile_count<T, true>), grid, dim3(kTileThreads), 0, s, g0, L0,
+                     (const T *)nullptr, (int64_t)0, (const int64_t *)nullptr,
+                     (const unsigned int *)nullptr, 0u, (unsigned int *)nullptr,
+                     (unsigned int *)nullptr, jobs, map);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tile_bounds_batch<T>, dim3(map.n), dim3(kScanThreads), 0, s, jobs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // the batch instances take their arguments from the job table
+  if (nc == 64)
+    hipLaunchKernelGGL((k_tile_sort<T, 64, 512, true>), grid, dim3(kTileThreads), 0, s, g0, L0,
+                       (const T *)nullptr, (int64_t)0, (const uint32_t *)nullptr,
+                       (const int64_t *)nullptr, (const unsigned int *)nullptr, 0u,
+                       (T *)nullptr, (uint32_t *)nullptr, (const int64_t *)nullptr,
+                       (const unsigned int *)nullptr, (unsigned int *)nullptr, jobs, map);
+  else
+    hipLaunchKernelGGL((k_tile_sort<T, 128, 256, true>), grid, dim3(kTileThreads), 0, s, g0,
+                       L0, (const T *)nullptr, (int64_t)0, (const uint32_t *)nullptr,
+                       (const int64_t *)nullptr, (const unsigned int *)nullptr, 0u,
+                       (T *)nullptr, (uint32_t *)nullptr, (const int64_t *)nullptr,
+                       (const unsigned int *)nullptr, (unsigned int *)nullptr, jobs, map);
+  return hipGetLastError();
+}
+
+template hipError_t tile_sort<double>(const TileSortJob<double> &, hipStream_t);
+template hipError_t tile_sort<float>(const TileSortJob<float> &, hipStream_t);
+template hipError_t tile_sort_batch<double>(const TileSortJob<double> *, const TileSortMap &,
+                                            int, int, int, unsigned int *, size_t,
+                                            hipStream_t);
+template hipError_t tile_sort_batch<float>(const TileSortJob<float> *, const TileSortMap &,
+                                           int, int, int, unsigned int *, size_t,
+                                           hipStream_t);
 
 hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
   if (c.ns <= 0 || c.n <= 0) return hipSuccess;

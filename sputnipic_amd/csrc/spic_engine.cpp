@@ -99,26 +99,45 @@ hipError_t ensure_workspace(spic_engine *e, size_t need, hipStream_t s) {
 // Sort species `is` into the other buffer and the other layout.  With
 // `src` given, the particles come densely from the caller's arrays instead.
 // A re-sort in SPIC_SORT_TILE mode is a tile sort.
+// The tile-sort job of species `is` from buffer `cur` in layout `segi` into
+// the other ones; count / own / fill at ws.
+template <typename T>
+spic::TileSortJob<T> tile_sort_job(const spic_engine *e, int is, int cur, int segi,
+                                   unsigned int *ws) {
+  const auto &sp = e->sp[is];
+  spic::TileSortJob<T> J;
+  std::memset(&J, 0, sizeof(J));
+  J.g = spic::kgrid<T>(&e->grd);
+  J.L = e->L;
+  J.L.slack = sp.slack;
+  const int from = cur, to = 1 - cur;
+  J.src = (const T *)sp.block[from];
+  J.dst = (T *)sp.block[to];
+  J.stride = sp.cap;
+  J.src_id = sp.id[from];
+  J.dst_id = sp.id[to];
+  J.src_seg = sp.seg[segi];
+  J.dst_seg = sp.seg[1 - segi];
+  J.src_cnt = sp.cnt[from];
+  J.dst_cnt = sp.cnt[to];
+  J.ovf_cap = sp.ovf_cap;
+  const size_t nt1 = (size_t)e->L.ntiles + 1;
+  J.count = ws;
+  J.own = ws + nt1;
+  J.fill = ws + 2 * nt1;
+  return J;
+}
+
 template <typename T>
 hipError_t tile_sort_species(spic_engine *e, int is, hipStream_t s) {
   auto &sp = e->sp[is];
-  spic::BinLayout L = e->L;
-  L.slack = sp.slack;
-  const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
-  const int from = sp.cur, to = 1 - sp.cur, lay_to = 1 - sp.segi;
-  size_t need = 0;
-  hipError_t err = spic::tile_sort<T>(g, L, nullptr, sp.cap, nullptr, nullptr, nullptr,
-                                      sp.ovf_cap, nullptr, nullptr, nullptr, nullptr,
-                                      nullptr, 0, &need, s);
+  hipError_t err =
+      ensure_workspace(e, 3 * ((size_t)e->L.ntiles + 1) * sizeof(unsigned int), s);
   if (err != hipSuccess) return err;
-  if ((err = ensure_workspace(e, need, s)) != hipSuccess) return err;
-  err = spic::tile_sort<T>(g, L, (const T *)sp.block[from], sp.cap, sp.id[from],
-                           sp.seg[sp.segi], sp.cnt[from], sp.ovf_cap, (T *)sp.block[to],
-                           sp.id[to], sp.seg[lay_to], sp.cnt[to], e->ws, e->ws_bytes,
-                           nullptr, s);
+  err = spic::tile_sort<T>(tile_sort_job<T>(e, is, sp.cur, sp.segi, (unsigned int *)e->ws), s);
   if (err != hipSuccess) return err;
-  sp.cur = to;
-  sp.segi = lay_to;
+  sp.cur = 1 - sp.cur;
+  sp.segi = 1 - sp.segi;
   e->sorts++;
   return hipSuccess;
 }
@@ -565,6 +584,16 @@ struct spic_batch {
   size_t arg_bytes = 0;
   spic::BatchMap tiles{}, ovf{}, nodes{}, counts{};
   int max_ns = 0;
+  // re-sorts of the whole batch in four launches when every member sorts
+  // by tile sort: device tables of the jobs (member species with
+  // particles) per (layout, buffer) state, their block map, and the jobs'
+  // count / own / fill words
+  bool tile_sorts = false;
+  void *sort_tab = nullptr;
+  size_t job_bytes = 0;
+  spic::TileSortMap sort_map{};
+  unsigned int *sort_ws = nullptr;
+  size_t sort_ws_words = 0, sort_ws_cap = 0;
 };
 
 namespace {
@@ -607,7 +636,70 @@ hipError_t build_tables(spic_batch *b, unsigned int *status) {
     b->gen[i] = e->gen;
   }
   b->status = status;
+
+  b->tile_sorts = true;
+  for (int i = 0; i < b->n; ++i)
+    if (b->e[i]->sort_mode != SPIC_SORT_TILE) b->tile_sorts = false;
+  if (!b->tile_sorts) return hipSuccess;
+  constexpr int kMaxJobs = spic::kMaxInstances * spic::kMaxSpecies;
+  const size_t jb = sizeof(spic::TileSortJob<T>);
+  size_t words = 0;
+  int nj = 0;
+  b->sort_map.start[0] = 0;
+  for (int i = 0; i < b->n; ++i) {
+    const spic_engine *e = b->e[i];
+    for (int is = 0; is < e->ns; ++is) {
+      if (!e->sp[is].loaded || e->sp[is].n == 0) continue;
+      words += 3 * ((size_t)e->L.ntiles + 1);
+      b->sort_map.start[nj + 1] = b->sort_map.start[nj] + e->L.ntiles + spic::kTileOvfBlocks;
+      ++nj;
+    }
+  }
+  b->sort_map.n = nj;
+  if (words > b->sort_ws_cap) {
+    if (b->sort_ws) (void)hipFree(b->sort_ws);
+    b->sort_ws = nullptr;
+    b->sort_ws_cap = 0;
+    if ((err = hipMalloc(&b->sort_ws, words * sizeof(unsigned int))) != hipSuccess) return err;
+    b->sort_ws_cap = words;
+  }
+  b->sort_ws_words = words;
+  if (!b->sort_tab) {
+    if ((err = hipMalloc(&b->sort_tab, 4 * kMaxJobs * jb)) != hipSuccess) return err;
+  }
+  std::vector<spic::TileSortJob<T>> jobs(4 * (size_t)kMaxJobs);
+  for (int st = 0; st < 4; ++st) {
+    size_t off = 0;
+    int j = 0;
+    for (int i = 0; i < b->n; ++i) {
+      const spic_engine *e = b->e[i];
+      for (int is = 0; is < e->ns; ++is) {
+        if (!e->sp[is].loaded || e->sp[is].n == 0) continue;
+        jobs[(size_t)st * kMaxJobs + j] =
+            tile_sort_job<T>(e, is, st & 1, st >> 1, b->sort_ws + off);
+        off += 3 * ((size_t)e->L.ntiles + 1);
+        ++j;
+      }
+    }
+  }
+  if ((err = hipMemcpy(b->sort_tab, jobs.data(), 4 * kMaxJobs * jb,
+                       hipMemcpyHostToDevice)) != hipSuccess)
+    return err;
+  b->job_bytes = jb;
   return hipSuccess;
+}
+
+// A re-sort of every loaded species of the batch, from `state`, by tile
+// sorts in four launches.
+template <typename T>
+hipError_t launch_batch_tile_sort(spic_batch *b, int state, hipStream_t s) {
+  constexpr int kMaxJobs = spic::kMaxInstances * spic::kMaxSpecies;
+  const spic::TileSortJob<T> *jobs =
+      (const spic::TileSortJob<T> *)((char *)b->sort_tab +
+                                     (size_t)state * kMaxJobs * b->job_bytes);
+  const spic::BinLayout &L = b->e[0]->L;
+  return spic::tile_sort_batch<T>(jobs, b->sort_map, L.tx, L.ty, L.tz, b->sort_ws,
+                                  b->sort_ws_words, s);
 }
 
 template <typename T>
@@ -728,7 +820,23 @@ int spic_batch_phase(spic_batch *b, void *stream) {
     err = b->prec == SPIC_F64 ? build_tables<double>(b, st) : build_tables<float>(b, st);
     if (err != hipSuccess) return hip_fail(err, "batch argument tables");
   }
-  if (sort_now) {
+  if (sort_now && b->tile_sorts) {
+    err = b->prec == SPIC_F64 ? launch_batch_tile_sort<double>(b, state, s)
+                              : launch_batch_tile_sort<float>(b, state, s);
+    if (err != hipSuccess) return hip_fail(err, "batch tile sort");
+    for (int i = 0; i < b->n; ++i) {
+      spic_engine *e = b->e[i];
+      std::lock_guard<std::mutex> lock(e->mu);
+      for (int is = 0; is < e->ns; ++is) {
+        auto &sp = e->sp[is];
+        if (!sp.loaded || sp.n == 0) continue;
+        sp.cur = 1 - sp.cur;
+        sp.segi = 1 - sp.segi;
+        e->sorts++;
+      }
+    }
+    state ^= 3;
+  } else if (sort_now) {
     // A sort moves each species to the other buffer and the other layout;
     // members in step stay in one state.
     int after = -1;
@@ -766,10 +874,10 @@ int spic_batch_phase(spic_batch *b, void *stream) {
 
 int spic_batch_destroy(spic_batch *b) {
   if (!b) return SPIC_OK;
-  if (b->tab) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(b->tab);
-  }
+  if (b->tab || b->sort_tab || b->sort_ws) (void)hipDeviceSynchronize();
+  if (b->tab) (void)hipFree(b->tab);
+  if (b->sort_tab) (void)hipFree(b->sort_tab);
+  if (b->sort_ws) (void)hipFree(b->sort_ws);
   delete b;
   return SPIC_OK;
 }

@@ -7,11 +7,14 @@ Each tile shape needs its own engines, so the caller supplies a builder:
 ``make_instances(tile)`` returns fresh instances (loaded from the same
 initial condition every time) with ``member()`` (the Batch member tuple),
 ``zero(stream)`` (zero the instance's moments) and ``close()``.  Every
-candidate runs ``warmup`` untimed and ``phases`` timed cycles (zero unless
+candidate starts from freshly sorted instances (every member engine is
+sorted first, so that no candidate inherits the order another one let
+decay) and runs ``warmup`` untimed and ``phases`` timed cycles (zero unless
 the batch writes the moments, batch phase) on one stream, timed with HIP
-events.  The tuner advances only its
-own instances; the caller builds the production instances afterwards with
-the winning settings.
+events; the schedule's sorts fall inside that span when ``warmup +
+phases`` exceeds the sort interval.  The tuner advances only its own
+instances; the caller builds the production instances afterwards with the
+winning settings.
 """
 from __future__ import annotations
 
@@ -91,6 +94,8 @@ def tune_batch(make_instances: Callable[[Tuple[int, int, int]], Sequence],
         insts = make_instances(tuple(tile))
         try:
             for se in sort_choices:
+                for x in insts:
+                    x.member()[0].sort(stream)
                 batch = Batch([x.member() for x in insts], sort_every=se,
                               set_moments=set_moments)
                 try:
