@@ -184,6 +184,41 @@ def test_batch_set_moments_keeps_global_deposits(prec):
         x.eng.close()
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_members_loaded_again_rebuild_the_tables(prec):
+    """Three phases (re-sorted every second phase), then every member's
+    species are loaded again -- member 0 from a denser initial condition,
+    the others from their own current particles -- and the batch rebuilds
+    its phase and sort tables at the next phase; three more phases match
+    the oracle."""
+    states = [_oracle(spec, prec) for spec in INSTANCES]
+    insts = [Instance(st, prec) for st in states]
+    batch = S.Batch([x.member() for x in insts], sort_every=2, set_moments=True)
+    for _ in range(3):
+        batch.phase()
+    for st in states:
+        for _ in range(3):
+            st.cycle()
+    _check([x.results() for x in insts], states, prec, "before the loads")
+    dense = dict(INSTANCES[0], ppc=(3, 3, 3))
+    states[0] = _oracle(dense, prec)
+    c = states[0].cfg
+    insts[0].parts = [S.Particles(p, c.qom[s_], prec, c.NiterMover)
+                      for s_, p in enumerate(states[0].parts)]
+    for x in insts:
+        for s_, p in enumerate(x.parts):
+            x.eng.load(s_, p)
+    for _ in range(3):
+        batch.phase()
+    for st in states:
+        for _ in range(3):
+            st.cycle()
+    _check([x.results() for x in insts], states, prec, "after the loads")
+    batch.close()
+    for x in insts:
+        x.eng.close()
+
+
 def test_batch_graph_replays_match_oracle():
     """Two phases (zero, batch phase) captured in one graph and replayed
     twice, each replay checked against the oracle after it."""
