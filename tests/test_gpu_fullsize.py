@@ -1,7 +1,9 @@
 """Properties at the benchmark's full size (GPU): the 128x64x64 Harris sheet
 on a 20x10x10 box with its four species at 5x5x5 particles per cell,
 262,144,000 particles generated on the device, through two engine phases,
-a tile sort and a third phase, in FP64 and in the FP32 form.
+a tile sort and a third phase, in FP64 and in the FP32 form; and the stress
+shape (eight species, 314,572,800 particles, FP32) through eleven phases
+with a tile sort before every fifth.
 
 No oracle finishes at this size; the checks are what the mathematics fixes
 whatever the size, on every particle and node:
@@ -39,14 +41,27 @@ def _close(got: float, want: float, scale: float, tol: float, what: str) -> None
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_full_size_phases_and_tile_sort_keep_the_invariants(prec):
+    _run(prec, S.harris_species(PPC), phases=3, sort_every=0, sort_after=2)
+
+
+def test_stress_shape_keeps_the_invariants():
+    """The stress shape: the four Harris species plus four x-beams (drift
+    +-0.1, thermal speed 0.02) at 1x5x5 per cell, 314,572,800 particles in
+    FP32, a tile sort before every fifth phase, eleven phases: the beams
+    cross tile faces every few phases, so the tile sorts move many particles
+    between segments."""
+    _run("f32", S.harris_species(PPC) + S.beam_species(), phases=11, sort_every=5,
+         sort_after=None)
+
+
+def _run(prec, species, phases, sort_every, sort_after):
     S.set_async(False)
     S.set_strict_fp(False)
     grd = S.Grid(*GRID, *BOX, prec)
     gem = S.GemParams()
     field = S.gem_fields(grd, gem)
-    species = S.harris_species(PPC)
     ns = len(species)
-    eng = S.Engine(grd, ns, set_moments=True)
+    eng = S.Engine(grd, ns, set_moments=True, sort_every=sort_every)
     counts, q0 = [], []
     for s, sp in enumerate(species):
         p = S.gem_particles(grd, gem, sp, s, seed=0)
@@ -55,14 +70,16 @@ def test_full_size_phases_and_tile_sort_keep_the_invariants(prec):
         eng.load(s, p)
         for d in p.d.values():
             d.free()
-    assert sum(counts) == 262_144_000
+    assert sum(counts) == {4: 262_144_000, 8: 314_572_800}[ns]
     moms = [S.InterpDensSpecies(grd) for _ in range(ns)]
     param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
-    eng.phase(field, moms, param)
-    eng.phase(field, moms, param)
-    eng.sort()
-    assert eng.stats()["overflow"] == 0
-    eng.phase(field, moms, param)
+    for ph in range(phases):
+        if ph == sort_after:
+            eng.sort()
+            assert eng.stats()["overflow"] == 0
+        eng.phase(field, moms, param)
+    if sort_every:
+        assert eng.stats()["sorts"] == ns * (1 + (phases - 1) // sort_every)
 
     tol = 1e-12 if prec == "f64" else 2e-6
     iv = float(grd.invVOL)
