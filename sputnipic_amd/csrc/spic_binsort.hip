@@ -588,14 +588,14 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
     uint32_t *dst_id_a, const int64_t *dst_seg_a, const unsigned int *own_a,
     unsigned int *fill_a, const TileSortJob<T> *__restrict__ jobs, TileSortMap map) {
   static_assert(KT * NC <= kTileStageBytes, "slot table exceeds the stage");
-  static_assert(NC <= 255, "cell index must fit a byte below the foreign mark");
-  constexpr unsigned char kForeign = 0xFF;
+  static_assert(NC <= 256, "counts of the cells before a cell must fit a byte");
+  constexpr uint16_t kForeign = 0xFFFF;  // also "no slot": foreign particles take none
   constexpr uint16_t kNoSlot = 0xFFFF;
   constexpr int kChunk = kTileStageBytes / (int)sizeof(T);
   __shared__ unsigned int s_count[NC], s_rank[NC], s_kmax, s_own;
   __shared__ unsigned int s_A[KT];
+  // each staged particle's cell (step 1), then its slot (step 2)
   __shared__ uint16_t s_slot[kTileStaged];
-  __shared__ unsigned char s_cell[kTileStaged];
   __shared__ __align__(16) unsigned char s_raw[kTileStageBytes];
   unsigned char(*s_T)[KT] = reinterpret_cast<unsigned char(*)[KT]>(s_raw);
   int t = blockIdx.x;
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
         cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
         const bool home = (int)tile == t;
         if (home) atomicAdd(&s_count[cit], 1u);
-        if (staged) s_cell[r] = home ? (unsigned char)cit : kForeign;
+        if (staged) s_slot[r] = home ? (uint16_t)cit : kForeign;
       }
     }
   }
@@ -686,6 +686,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
   const int kmax = (int)(s_kmax < (unsigned)KT ? s_kmax : (unsigned)KT);
   for (int k = tid; k < kmax; k += kTileThreads) {
     unsigned int a = 0, before = 0;
+#pragma unroll 8
     for (int c = 0; c < NC; ++c) {
       const unsigned int n = s_count[c];
       a += n < (unsigned)k ? n : (unsigned)k;
@@ -707,7 +708,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
     unsigned int c = kForeign;
     uint32_t tile = 0;
     if (in) {
-      if (staged) c = s_cell[r];
+      if (staged) c = s_slot[r];
       if (!staged || c == kForeign) {
         const int64_t j = v.slot(r);
         int cx, cy, cz;
@@ -732,6 +733,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
       pos = s_A[k] + s_T[c][k];
     } else {
       pos = 0;
+#pragma unroll 4
       for (int cc = 0; cc < NC; ++cc) {
         const unsigned int n = s_count[cc];
         pos += n < k ? n : k;
