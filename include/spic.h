@@ -314,6 +314,10 @@ int spic_engine_store(spic_engine *eng, int is, spic_species *sp,
 int spic_engine_phase(spic_engine *eng, const spic_field *field,
                       const spic_moments *mom, const spic_push *push,
                       void *stream);
+/* interpP2G of every loaded species from its binned particles, without a
+ * push (src/Mover.cpp:385-572): the moments are added into mom[is] (or
+ * written, with set_moments).  Not a phase for the sort schedule. */
+int spic_engine_deposit(spic_engine *eng, const spic_moments *mom, void *stream);
 /* Particle snapshot of species `is` in load order: x y z u v w as
  * big-endian IEEE float32 words (values rounded to float) into six device
  * arrays of nop uint32 -- the datasets HDF5_Write_Particles writes per

@@ -146,6 +146,7 @@ SIGNATURES = {
     "spic_engine_destroy": (C.c_int, [_P]),
     "spic_engine_load": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
     "spic_engine_store": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
+    "spic_engine_deposit": (C.c_int, [_P, C.POINTER(spic_moments), _P]),
     "spic_engine_phase": (C.c_int, [_P, C.POINTER(spic_field), C.POINTER(spic_moments),
                                     C.POINTER(spic_push), _P]),
     "spic_engine_sort": (C.c_int, [_P, _P]),

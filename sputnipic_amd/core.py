@@ -643,6 +643,13 @@ class Engine:
         check(lib().spic_engine_phase(self.ptr, C.byref(f), moms, C.byref(p),
                                       stream.ptr if stream else None), "spic_engine_phase")
 
+    def deposit(self, ids: Sequence[InterpDensSpecies],
+                stream: Optional["Stream"] = None) -> None:
+        """interpP2G of every loaded species, no push (spic_engine_deposit)."""
+        moms = (spic_moments * self.ns)(*[ids[i].c_struct() for i in range(self.ns)])
+        check(lib().spic_engine_deposit(self.ptr, moms, stream.ptr if stream else None),
+              "spic_engine_deposit")
+
     def sort(self, stream: Optional["Stream"] = None) -> None:
         check(lib().spic_engine_sort(self.ptr, stream.ptr if stream else None),
               "spic_engine_sort")

@@ -113,6 +113,7 @@ struct BinnedArgs {
   int ns;
   int lds_fp64;            // FP32 particles: LDS moment block in FP64
   int n_ovf_blocks;        // workgroups for the overflow regions
+  int push_off;            // 1: deposit only (interpP2G); no push, no field read
   BinnedSpecies<T> sp[kMaxSpecies];
   // status words: [0] mover iterate out of the grid, [1] deposit cell out
   // of the grid, [2] overflow region full

@@ -443,7 +443,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       s_zn[b - BX - BY] =
           (blk.c0z + b - BX - BY < nzn) ? gmem(a.g.zn)[blk.c0z + b - BX - BY] : T(0);
   }
-  for (int b = tid; b < NB; b += NT) {
+  for (int b = a.push_off ? NB : tid; b < NB; b += NT) {
     const int bz = b % BZ, by = (b / BZ) % BY, bx = b / (BZ * BY);
     const int gx = blk.c0x + bx, gy = blk.c0y + by, gz = blk.c0z + bz;
     T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -506,8 +506,9 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       }
       int ix = 2, iy = 2, iz = 2, dest = -1;
       if (active) {
-        push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
-                                           bad_move, n_lg, n_gg);
+        if (!a.push_off)
+          push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
+                                             bad_move, n_lg, n_gg);
         ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
         iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
         iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
@@ -747,7 +748,8 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
       T u = p[3 * st], v = p[4 * st], w = p[5 * st];
       const T q = p[6 * st];
       const uint32_t pid = gmem(sp.id_in)[i];
-      push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
+      if (!a.push_off)
+        push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
                                        bad_move, n_lg, n_gg);
       int ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
       int iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);

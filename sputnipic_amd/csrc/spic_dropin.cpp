@@ -34,6 +34,8 @@ int g_resident = 0;
 struct GridMirror {
   spic_grid g;
   void *xn = nullptr, *yn = nullptr, *zn = nullptr;
+  spic_engine *eng = nullptr;  // single-species engine for the deposits
+  bool eng_failed = false;
 };
 struct FieldMirror {
   void *d[6] = {};
@@ -229,6 +231,25 @@ spic_push push_of(const parameters *param) {
 
 size_t nodes(const grid *grd) { return (size_t)grd->nxn * grd->nyn * grd->nzn; }
 
+// The grid's single-species engine: interpP2G and mover_interp bin the
+// caller's particles into it and deposit through the LDS tiles of the
+// binned kernels instead of the caller-order kernels' global atomics.  Each
+// call loads the particles anew (they are the caller's, in the caller's
+// order) and finishes with a blocking copy of the moments, so calls do not
+// overlap in it.  Without an engine (its creation failed) the calls use the
+// caller-order kernels.
+spic_engine *grid_engine(GridMirror *gm) {
+  if (!gm->eng && !gm->eng_failed) {
+    spic_engine_opts o;
+    std::memset(&o, 0, sizeof(o));
+    if (spic_engine_create(&gm->eng, &gm->g, 1, &o) != SPIC_OK) {
+      gm->eng = nullptr;
+      gm->eng_failed = true;
+    }
+  }
+  return gm->eng;
+}
+
 }  // namespace
 
 int mover_PC(struct particles *part, struct EMfield *field, struct grid *grd,
@@ -261,7 +282,15 @@ void interpP2G(struct particles *part, struct interpDensSpecies *ids,
   if (rc == SPIC_OK) rc = part_mirror(part, &pm, &sp);
   if (rc == SPIC_OK) rc = mom_mirror(ids, nodes(grd), &mm, &mom);
   if (rc == SPIC_OK) rc = spic_zero_moments(&gm->g, &mom, stream());
-  if (rc == SPIC_OK) rc = spic_interp_p2g(&gm->g, &sp, &mom, stream());
+  if (rc == SPIC_OK) {
+    spic_engine *e = grid_engine(gm);
+    if (e) {
+      rc = spic_engine_load(e, 0, &sp, stream());
+      if (rc == SPIC_OK) rc = spic_engine_deposit(e, &mom, stream());
+    } else {
+      rc = spic_interp_p2g(&gm->g, &sp, &mom, stream());
+    }
+  }
   if (rc == SPIC_OK) rc = mom_add_to_host(ids, mm);
   report("interpP2G", rc);
 }
@@ -283,7 +312,14 @@ int mover_interp(struct particles *part, struct EMfield *field,
   if (rc == SPIC_OK) rc = spic_zero_moments(&gm->g, &mom, stream());
   if (rc == SPIC_OK) {
     const spic_push p = push_of(param);
-    rc = spic_mover_interp(&gm->g, &f, &sp, &mom, &p, stream());
+    spic_engine *e = grid_engine(gm);
+    if (e) {
+      rc = spic_engine_load(e, 0, &sp, stream());
+      if (rc == SPIC_OK) rc = spic_engine_phase(e, &f, &mom, &p, stream());
+      if (rc == SPIC_OK) rc = spic_engine_store(e, 0, &sp, stream());
+    } else {
+      rc = spic_mover_interp(&gm->g, &f, &sp, &mom, &p, stream());
+    }
   }
   if (rc == SPIC_OK) rc = part_written(part, pm, false);
   if (rc == SPIC_OK) rc = mom_add_to_host(ids, mm);
@@ -339,6 +375,7 @@ int spic_dropin_release_all(void) {
   std::lock_guard<std::mutex> lock(g_mu);
   spic_synchronize();
   for (auto &kv : g_grids) {
+    if (kv.second.eng) spic_engine_destroy(kv.second.eng);
     spic_free(kv.second.xn);
     spic_free(kv.second.yn);
     spic_free(kv.second.zn);

@@ -230,14 +230,17 @@ int fill_args(const spic_engine *e, const spic_field *field, const spic_moments 
   return k;
 }
 
+// One phase of every loaded species; with push_off, the deposit alone
+// (the particles are copied to the other buffer unchanged).
 template <typename T>
 hipError_t launch_phase(spic_engine *e, const spic_field *field,
                         const spic_moments *mom, const spic_push *push,
-                        unsigned int *status, hipStream_t s) {
+                        unsigned int *status, hipStream_t s, int push_off = 0) {
   spic::BinnedArgs<T> a;
   int idx[spic::kMaxSpecies];
   const int k = fill_args<T>(e, field, mom, push, status, -1, -1, a, idx);
   if (k == 0) return hipSuccess;
+  a.push_off = push_off;
   spic::BinnedReduceArgs r;
   std::memset(&r, 0, sizeof(r));
   spic::CountArrays zc;
@@ -528,6 +531,30 @@ int spic_engine_phase(spic_engine *e, const spic_field *field,
   err = e->grd.prec == SPIC_F64 ? launch_phase<double>(e, field, mom, push, st, s)
                                 : launch_phase<float>(e, field, mom, push, st, s);
   e->phases++;
+  return spic::end_call(s, st, err);
+}
+
+int spic_engine_deposit(spic_engine *e, const spic_moments *mom, void *stream) {
+  if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  if (!mom) return fail(SPIC_EINVAL, "moments are NULL");
+  int rc;
+  for (int is = 0; is < e->ns; ++is) {
+    if (!e->sp[is].loaded) continue;
+    if ((rc = spic::check_moments(&mom[is]))) return rc;
+  }
+  std::lock_guard<std::mutex> lock(e->mu);
+  hipStream_t s = (hipStream_t)stream;
+  // neither is read: the kernels skip the push and the field stage
+  spic_field f0;
+  std::memset(&f0, 0, sizeof(f0));
+  spic_push p0;
+  std::memset(&p0, 0, sizeof(p0));
+  p0.c = 1.0;
+  unsigned int *st;
+  if ((rc = spic::begin_call(s, &st))) return rc;
+  hipError_t err = e->grd.prec == SPIC_F64
+                       ? launch_phase<double>(e, &f0, mom, &p0, st, s, 1)
+                       : launch_phase<float>(e, &f0, mom, &p0, st, s, 1);
   return spic::end_call(s, st, err);
 }
 

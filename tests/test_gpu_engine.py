@@ -304,6 +304,54 @@ def test_engine_fast_against_f64_oracle(prec):
                                                                    else 1e-5), k
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_deposit_is_interp_p2g(prec):
+    """spic_engine_deposit: interpP2G of the loaded particles without a push,
+    against the oracle's interpP2G of the same arrays; the particles come
+    back unchanged, the call is not a phase, and a phase after it is the
+    oracle's mover_PC + interpP2G."""
+    st = oracle_small(prec)
+    c = st.cfg
+    S.set_strict_fp(True)
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
+    dt = np.float64 if prec == "f64" else np.float32
+    field = S.EMField(grd, {n: st.fields[n].astype(dt) for n in O.FIELD_NAMES})
+    parts = [S.Particles(p, c.qom[s], prec, c.NiterMover) for s, p in enumerate(st.parts)]
+    ids = [S.InterpDensSpecies(grd) for _ in range(c.ns)]
+    param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
+                         PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
+    eng = S.Engine(grd, c.ns)
+    for s, p in enumerate(parts):
+        eng.load(s, p)
+    before = [{k: v.copy() for k, v in p.items()} for p in st.parts]
+    S.setZeroSpeciesDensities(ids, grd, c.ns)
+    eng.deposit(ids)
+    assert eng.stats()["phases"] == 0
+    tol = 1e-13 if prec == "f64" else 1e-5
+    for s in range(c.ns):
+        st.zero_moments(s)
+        st.interp_p2g(s)
+        got = ids[s].download()
+        for n in O.MOMENT_NAMES:
+            assert_close(got[n], st.moments[s][n], tol, f"deposit s{s} {n}")
+    S.setZeroSpeciesDensities(ids, grd, c.ns)
+    eng.phase(field, ids, param)
+    for s, p in enumerate(parts):
+        eng.store(s, p)
+    for s in range(c.ns):
+        for k in O.PARTICLE_NAMES:
+            st.parts[s][k][:] = before[s][k]
+    oracle_phases(st, 1)
+    got_p = [p.download() for p in parts]
+    got_m = [m.download() for m in ids]
+    for s in range(c.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"phase s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], tol, f"phase s{s} {n}")
+    eng.close()
+
+
 def test_engine_conserves_charge():
     st = oracle_small("f64")
     got_p, got_m, _ = run_engine(st, "f64", phases=1)
