@@ -36,6 +36,7 @@ struct spic_engine {
     int niter = 3, nsub = 1;
     int64_t slack = 0;             // free slots per segment after a sort
     int64_t cap = 0;               // slots per buffer
+    int64_t alloc_cap = 0;         // slots the buffers were allocated for
     unsigned int ovf_cap = 0;      // overflow slots (= n: nothing is ever lost)
     void *block[2] = {};           // two buffers of x y z u v w q, one block each
     void *buf[2][7] = {};          // the components: block[b] + c * cap
@@ -71,14 +72,26 @@ hipError_t dmalloc(spic_engine *e, void **p, size_t bytes) {
 }
 
 void free_species(spic_engine *e, spic_engine::Species &s) {
+  const size_t nt1 = (size_t)e->L.ntiles + 1;
   for (int b = 0; b < 2; ++b) {
-    if (s.block[b]) (void)hipFree(s.block[b]);
-    if (s.id[b]) (void)hipFree(s.id[b]);
-    if (s.cnt[b]) (void)hipFree(s.cnt[b]);
-    if (s.seg[b]) (void)hipFree(s.seg[b]);
+    if (s.block[b]) {
+      (void)hipFree(s.block[b]);
+      e->bytes -= (size_t)e->esz * 7 * s.alloc_cap;
+    }
+    if (s.id[b]) {
+      (void)hipFree(s.id[b]);
+      e->bytes -= sizeof(uint32_t) * s.alloc_cap;
+    }
+    if (s.cnt[b]) {
+      (void)hipFree(s.cnt[b]);
+      e->bytes -= sizeof(unsigned int) * 3 * nt1;
+    }
+    if (s.seg[b]) {
+      (void)hipFree(s.seg[b]);
+      e->bytes -= sizeof(int64_t) * nt1;
+    }
   }
   s = spic_engine::Species();
-  (void)e;
 }
 
 hipError_t ensure_workspace(spic_engine *e, size_t need, hipStream_t s) {
@@ -405,14 +418,15 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
   // segments (n + nt * slack) then the overflow region (n)
   const int64_t cap = 2 * n + nt * slack;
   hipError_t err;
-  if (!sp.loaded || sp.cap != cap) {
+  // storage is kept while it holds `cap` slots (a smaller species loaded
+  // after a larger one reuses it with the smaller component stride)
+  if (!sp.block[0] || cap > sp.alloc_cap) {
     if ((err = hipDeviceSynchronize()) != hipSuccess) return hip_fail(err, "engine load");
     free_species(e, sp);
+    sp.alloc_cap = cap;
     for (int b = 0; b < 2; ++b) {
       if ((err = dmalloc(e, &sp.block[b], (size_t)e->esz * 7 * cap)) != hipSuccess)
         return hip_fail(err, "engine particle storage");
-      for (int a = 0; a < 7; ++a)
-        sp.buf[b][a] = (char *)sp.block[b] + (size_t)e->esz * a * cap;
       if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
         return hip_fail(err, "engine id storage");
       if ((err = dmalloc(e, (void **)&sp.cnt[b], sizeof(unsigned int) * 3 * (nt + 1))) !=
@@ -422,6 +436,9 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
         return hip_fail(err, "engine tile tables");
     }
   }
+  for (int b = 0; b < 2; ++b)
+    for (int a = 0; a < 7; ++a)
+      sp.buf[b][a] = (char *)sp.block[b] + (size_t)e->esz * a * cap;
   sp.n = n;
   sp.cap = cap;
   sp.slack = slack;

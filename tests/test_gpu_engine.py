@@ -352,6 +352,46 @@ def test_engine_deposit_is_interp_p2g(prec):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_species_storage_grows_and_is_reused(prec):
+    """One species slot loaded three times -- a small species, an eight
+    times larger one (the storage grows), the small one again (the storage
+    is kept, with the smaller component stride) -- each followed by a phase
+    that matches the oracle."""
+    S.set_strict_fp(True)
+    small = O.GemConfig(npcelx=[1] * 4, npcely=[1] * 4, npcelz=[1] * 4)
+    big = O.GemConfig(npcelx=[2] * 4, npcely=[2] * 4, npcelz=[2] * 4)
+    eng, grd, used = None, None, []
+    tol = 1e-13 if prec == "f64" else 1e-5
+    for cfg in (small, big, small):
+        st = O.State(cfg, prec)
+        c = st.cfg
+        if eng is None:
+            grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
+            eng = S.Engine(grd, 1)
+        dt = np.float64 if prec == "f64" else np.float32
+        field = S.EMField(grd, {n: st.fields[n].astype(dt) for n in O.FIELD_NAMES})
+        part = S.Particles(st.parts[0], c.qom[0], prec, c.NiterMover)
+        ids = [S.InterpDensSpecies(grd)]
+        param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
+                             PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
+        eng.load(0, part)
+        used.append(eng.stats()["device_bytes"])
+        S.setZeroSpeciesDensities(ids, grd, 1)
+        eng.phase(field, ids, param)
+        eng.store(0, part)
+        st.mover_pc(0)
+        st.zero_moments(0)
+        st.interp_p2g(0)
+        got_p, got_m = part.download(), ids[0].download()
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[k], st.parts[0][k], f"n={len(st.parts[0]['x'])} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[n], st.moments[0][n], tol, f"n={len(st.parts[0]['x'])} {n}")
+    assert used[0] < used[1] == used[2]
+    eng.close()
+
+
 def test_engine_conserves_charge():
     st = oracle_small("f64")
     got_p, got_m, _ = run_engine(st, "f64", phases=1)
