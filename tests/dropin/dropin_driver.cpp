@@ -7,7 +7,10 @@
 //   int32  nxc ny nzc ns niter nops strict resident sort_cps perx pery perz
 //   float64 Lx Ly Lz dt c
 //   int32  ops[nops]        1 mover_PC, 2 zero+interpP2G, 3 zero+mover_interp,
-//                           4 particle_sort  (each over all species)
+//                           4 particle_sort  (each over all species),
+//                           5 zero+interpP2G with one thread per species at
+//                             once, as the reference's OpenMP team calls it
+//                             (src/sputniPIC_CPU.cpp:250-256)
 //   FP     fields[6][Nn]    Ex Ey Ez Bxn Byn Bzn
 //   per species: int64 n; float64 qom; FP arrays x y z u v w q [n]
 // Output file: per species FP x y z u v w q [n], then FP moments[10][Nn].
@@ -18,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "spic.h"
@@ -143,6 +147,19 @@ int main(int argc, char **argv) {
           die("mover_interp failed");
         if (spic_dropin_last_status() != 0) die("deposit failed");
       }
+    } else if (op == 5) {
+      for (int is = 0; is < ns; is++)
+        std::fill(mstore[is].begin(), mstore[is].end(), (FPinterp)0);
+      std::vector<int> status(ns, -1);
+      std::vector<std::thread> team;
+      for (int is = 0; is < ns; is++)
+        team.emplace_back([&, is]() {
+          interpP2G(&part[is], &ids[is], &grd);
+          status[is] = spic_dropin_last_status();  // the calling thread's status
+        });
+      for (auto &t : team) t.join();
+      for (int is = 0; is < ns; is++)
+        if (status[is] != 0) die("threaded interpP2G failed");
     } else if (op == 4) {
       for (int is = 0; is < ns; is++) {
         particle_sort(&param, &part[is], &grd);

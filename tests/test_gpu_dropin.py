@@ -88,6 +88,24 @@ def test_dropin_reference_cycle(prec, resident):
             assert_close(moms[s][n], st.moments[s][n], tol, f"s{s} {n}")
 
 
+@pytest.mark.parametrize("resident", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dropin_interp_from_a_thread_team(prec, resident):
+    """The reference cycle with interpP2G called from one thread per species
+    at once, as the reference's OpenMP team calls it: the same results as
+    the sequential calls."""
+    st = small_state(prec)
+    parts, moms = run_driver(st, prec, [1, 5, 1, 5], resident=resident)
+    st.cycle()
+    st.cycle()
+    tol = 1e-13 if prec == "f64" else 1e-5
+    for s in range(st.cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(parts[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(moms[s][n], st.moments[s][n], tol, f"s{s} {n}")
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_dropin_fused_and_sort(prec):
     st = small_state(prec)
