@@ -400,7 +400,7 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
 // A: type of the LDS moment block (T, or double for FP32 particles); the
 // per-thread register sums are in the particle precision T.  FP32 kernels
 // ask for three waves per SIMD (at most 168 VGPRs).
-template <typename T, typename A, int TX, int TY, int TZ>
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true>
 __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const int tile) {
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
@@ -443,7 +443,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       s_zn[b - BX - BY] =
           (blk.c0z + b - BX - BY < nzn) ? gmem(a.g.zn)[blk.c0z + b - BX - BY] : T(0);
   }
-  for (int b = a.push_off ? NB : tid; b < NB; b += NT) {
+  for (int b = PUSH ? tid : NB; b < NB; b += NT) {  // fields: for the push only
     const int bz = b % BZ, by = (b / BZ) % BY, bx = b / (BZ * BY);
     const int gx = blk.c0x + bx, gy = blk.c0y + by, gz = blk.c0z + bz;
     T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -506,7 +506,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       }
       int ix = 2, iy = 2, iz = 2, dest = -1;
       if (active) {
-        if (!a.push_off)
+        if constexpr (PUSH)
           push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
                                              bad_move, n_lg, n_gg);
         ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
@@ -705,11 +705,13 @@ __device__ __forceinline__ int batch_instance(const BatchMap &map, int b) {
   return i;
 }
 
-template <typename T, typename A, int TX, int TY, int TZ>
+// PUSH false: the deposit alone (spic_engine_deposit), in its own
+// instances so that the phase kernels carry no test for it.
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true>
 __global__ __launch_bounds__(4 * TX * TY * TZ)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
 void k_binned_phase(BinnedArgs<T> a) {
-  binned_phase_tile<T, A, TX, TY, TZ>(a, blockIdx.x);
+  binned_phase_tile<T, A, TX, TY, TZ, PUSH>(a, blockIdx.x);
 }
 
 // Every instance of a batch in one grid: instance i owns blocks
@@ -726,7 +728,7 @@ void k_binned_phase_batch(const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
 // and deposits, re-binned like the tile kernel's.
 // Part `part` of `nparts` (workgroups of 256 threads) takes every
 // (nparts * 256)-th particle.
-template <typename T, int TX, int TY, int TZ>
+template <typename T, int TX, int TY, int TZ, bool PUSH = true>
 __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int part,
                                                      int nparts) {
   Block<T, 1, 1, 1> none;
@@ -748,9 +750,9 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
       T u = p[3 * st], v = p[4 * st], w = p[5 * st];
       const T q = p[6 * st];
       const uint32_t pid = gmem(sp.id_in)[i];
-      if (!a.push_off)
+      if constexpr (PUSH)
         push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
-                                       bad_move, n_lg, n_gg);
+                                         bad_move, n_lg, n_gg);
       int ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
       int iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
       int iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
@@ -774,9 +776,9 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
   }
 }
 
-template <typename T, int TX, int TY, int TZ>
+template <typename T, int TX, int TY, int TZ, bool PUSH = true>
 __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
-  binned_overflow_part<T, TX, TY, TZ>(a, blockIdx.x, gridDim.x);
+  binned_overflow_part<T, TX, TY, TZ, PUSH>(a, blockIdx.x, gridDim.x);
 }
 
 template <typename T, int TX, int TY, int TZ>
@@ -787,18 +789,18 @@ __global__ __launch_bounds__(256) void k_binned_overflow_batch(
                                       map.start[i + 1] - map.start[i]);
 }
 
-template <typename T, typename A>
+template <typename T, typename A, bool PUSH>
 hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
   if (a.tx == 4 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4>), dim3(a.ntiles), dim3(256), 0,
+    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH>), dim3(a.ntiles), dim3(256), 0,
                        s, a);
-    hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4>), dim3(a.n_ovf_blocks), dim3(256),
-                       0, s, a);
+    hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4, PUSH>), dim3(a.n_ovf_blocks),
+                       dim3(256), 0, s, a);
   } else if (a.tx == 8 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4>), dim3(a.ntiles), dim3(512), 0,
+    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH>), dim3(a.ntiles), dim3(512), 0,
                        s, a);
-    hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4>), dim3(a.n_ovf_blocks), dim3(256),
-                       0, s, a);
+    hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4, PUSH>), dim3(a.n_ovf_blocks),
+                       dim3(256), 0, s, a);
   } else {
     return hipErrorInvalidValue;
   }
@@ -808,8 +810,12 @@ hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
 template <typename T>
 hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
   if (a.ntiles <= 0 || a.ns <= 0) return hipSuccess;
-  return (sizeof(T) == 4 && a.lds_fp64) ? launch_binned_shape<T, double>(a, s)
-                                        : launch_binned_shape<T, T>(a, s);
+  const bool lds64 = sizeof(T) == 4 && a.lds_fp64;
+  if (a.push_off)
+    return lds64 ? launch_binned_shape<T, double, false>(a, s)
+                 : launch_binned_shape<T, T, false>(a, s);
+  return lds64 ? launch_binned_shape<T, double, true>(a, s)
+               : launch_binned_shape<T, T, true>(a, s);
 }
 
 template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);
