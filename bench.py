@@ -867,9 +867,9 @@ def main() -> int:
         print(f"copy probe: {n * 4} bytes read and {n * 4} bytes written", file=sys.stderr)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus}: this build measures one GPU only; the path "
-              "would shard by particles or by x-slabs (DESIGN.md), which is not built "
-              "here.", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus}: only one GPU is measured; the path would "
+              "shard by particles (one moment all-reduce per cycle) or by x-slabs (halo "
+              "folds and particle migration), which is not built.", file=sys.stderr)
         return 2
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: multi-rank launch is not supported (one GPU only)", file=sys.stderr)
