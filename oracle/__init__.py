@@ -1,8 +1,9 @@
 """CPU oracle for the sputniPIC particle hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker: the product
-(``sputnipic_amd``) never imports, links or calls it.
+``bench.py`` may import this package, and only to check against or to time
+as the CPU baseline: the product (``sputnipic_amd``) never imports, links or
+calls it.
 
 Two libraries are wrapped:
 
@@ -44,8 +45,8 @@ Two libraries are wrapped:
   ``InterpDensNet.cpp``, ``EMfield.cpp``, ``BC.cpp``, ``MaxwellSolver.cpp``,
   ``Basic.cpp`` and ``Solvers.cpp`` compiled unmodified
   from the reference checkout (``oracle/Makefile``) with the test harness in
-  ``ref_harness.cpp``.  Present only where the reference checkout exists
-  (this build container); never on the GPU machines.
+  ``ref_harness.cpp``.  Present only where the reference checkout exists at
+  build time; never where the GPU tests run without it.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * spic_oracle.c -- CPU oracle for the sputniPIC particle hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle, never the product:
  * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
  * may load the library built from it.  The product path (sputnipic_amd/)
  * never links or calls it.
