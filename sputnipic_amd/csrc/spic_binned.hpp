@@ -122,7 +122,7 @@ struct BinnedArgs {
   // deposits with global atomics, [2] stencil gathers from LDS, [3]
   // deposits through LDS (bucket or LDS atomics)
   unsigned long long *counters;
-  // per-tile moment partials [tile][species][moment][block node] in the
+  // per-tile moment partials [tile][species][block node][moment] in the
   // particle precision (PartialT<T>), written for every tile-species whose
   // input segment is not empty
   void *partials;

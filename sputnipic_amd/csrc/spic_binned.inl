@@ -671,8 +671,13 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     }
     __syncthreads();
     using P = PartialT<T>;
+    // stored node-major (the ten moments of a block node together), the
+    // order in which the reduce reads them; the store stays contiguous
     SPIC_GLOBAL P *dst = gmem((P *)a.partials) + ((size_t)tile * a.ns + s) * (10 * NB);
-    for (int i = tid; i < 10 * NB; i += NT) dst[i] = (P)s_mom[i];
+    for (int i = tid; i < 10 * NB; i += NT) {
+      const int b = i / 10, m = i - 10 * b;
+      dst[i] = (P)s_mom[m * NB + b];
+    }
     if (tid == 0) {
       // the reserved block [stay_base, stay_base + nin) holds s_stay
       // particles; its remainder inside the segment stays unused

@@ -286,10 +286,11 @@ __device__ __forceinline__ void reduce_node(int n, int s, int nyn, int nzn, int 
         if (!gmem(pflag)[(size_t)tile * ns + s]) continue;
         any = true;
         const int b = ((gx - tx * TX) * BY + (gy - ty * TY)) * BZ + (gz - tz * TZ);
-        const SPIC_GLOBAL P *src = gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + b;
+        const SPIC_GLOBAL P *src =
+            gmem(partials) + ((size_t)tile * ns + s) * (10 * NB) + (size_t)b * 10;
 #pragma unroll
         for (int m = 0; m < 10; ++m) {
-          const double v = (double)src[m * NB];
+          const double v = (double)src[m];
           const double t = sum[m] + v;
           cmp[m] += fabs(sum[m]) >= fabs(v) ? (sum[m] - t) + v : (v - t) + sum[m];
           sum[m] = t;
