@@ -143,6 +143,30 @@ def test_engine_sort_modes_keep_trajectories(prec, tile, sort_mode):
                          f"s{s} {n}")
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_tile_sort_of_a_crowded_tile(prec):
+    """One 4x4x4-cell tile holding 729 particles per cell per species
+    (46,656): more than the tile sort stages in LDS (it writes each particle
+    at its slot directly) and cells ranked beyond the slot tables' 512
+    columns (slots from the loop over the cells); re-sorted every phase,
+    the trajectories stay exact."""
+    cfg = O.GemConfig(nxc=4, nyc=4, nzc=4, ns=2, npcelx=[9] * 4, npcely=[9] * 4,
+                      npcelz=[9] * 4)
+    st = O.State(cfg, prec)
+    assert len(st.parts[0]["x"]) == 64 * 729
+    S.set_strict_fp(True)
+    got_p, got_m, stats = run_engine(st, prec, phases=3, sort_every=1,
+                                     sort_mode=S.SPIC_SORT_TILE)
+    oracle_phases(st, 3)
+    assert stats["sorts"] == cfg.ns * 3
+    for s in range(cfg.ns):
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n], 1e-13 if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
 @pytest.mark.parametrize("sort_mode", [S.SPIC_SORT_TILE, S.SPIC_SORT_RADIX])
 def test_engine_sort_places_crossing_particles_in_their_tiles(sort_mode):
     """gem_hot without fields, at rest, except species 0's last cell layer
