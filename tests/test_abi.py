@@ -123,3 +123,15 @@ def test_bad_species_is_rejected():
     g, f, s, p = _fake_args(niter=0)
     assert L.spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), C.byref(p),
                            None) == _capi.SPIC_EINVAL
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without libspic.so the package raises instead of computing anything
+    another way: there is no fallback path."""
+    import sputnipic_amd as S
+    monkeypatch.setattr(_capi, "_lib", None)
+    monkeypatch.setattr(_capi, "LIB_PATH", os.path.join(ROOT, "no-such-dir", "libspic.so"))
+    with pytest.raises(RuntimeError, match="libspic.so not built"):
+        _capi.lib()
+    with pytest.raises(RuntimeError, match="libspic.so not built"):
+        S.device_count()
