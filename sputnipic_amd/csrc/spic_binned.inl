@@ -490,20 +490,34 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
     const int64_t nbatches = (nin + NT - 1) / NT;
+    // this thread's particle of batch j (x y z u v w q and its id); with
+    // PREFETCH (FP32, which has the registers for it at three waves per
+    // SIMD; FP64 would spill) batch j + 1 is loaded while batch j is pushed
+    constexpr bool PREFETCH = sizeof(T) == 4;
+    T nxt[7];
+    uint32_t nxt_id = 0;
+    auto load_particle = [&](int64_t r) {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) nxt[c] = T(0);
+      nxt_id = 0;
+      if (r < nin) {
+        const int64_t i = in.slot(r);
+        const SPIC_GLOBAL T *p = gmem(sp.in) + i;
+        const int64_t st = sp.stride;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) nxt[c] = p[c * st];
+        nxt_id = gmem(sp.id_in)[i];
+      }
+    };
+    if (PREFETCH) load_particle(tid);
     for (int64_t j = 0; j < nbatches; ++j) {
       const int64_t r = j * NT + tid;
       const bool active = r < nin;
       const int buf = (int)(j & 1);
-      T x = T(0), y = T(0), z = T(0), u = T(0), v = T(0), w = T(0), q = T(0);
-      uint32_t pid = 0;
-      if (active) {
-        const int64_t i = in.slot(r);
-        const SPIC_GLOBAL T *p = gmem(sp.in) + i;
-        const int64_t st = sp.stride;
-        x = p[0]; y = p[st]; z = p[2 * st];
-        u = p[3 * st]; v = p[4 * st]; w = p[5 * st]; q = p[6 * st];
-        pid = gmem(sp.id_in)[i];
-      }
+      if (!PREFETCH) load_particle(r);
+      T x = nxt[0], y = nxt[1], z = nxt[2], u = nxt[3], v = nxt[4], w = nxt[5], q = nxt[6];
+      uint32_t pid = nxt_id;
+      if (PREFETCH && j + 1 < nbatches) load_particle(r + NT);
       int ix = 2, iy = 2, iz = 2, dest = -1;
       if (active) {
         if constexpr (PUSH)
