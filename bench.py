@@ -836,7 +836,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=10)
+    ap.add_argument("--cpu-cycles", type=int, default=40)
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
