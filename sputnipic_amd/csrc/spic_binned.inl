@@ -96,6 +96,10 @@ struct Block {
   static constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, NB = BX * BY * BZ;
   const T *fld;
   const T *xn, *yn, *zn;
+  // Ex Ey Ez Bx By Bz and the node coordinates xn yn zn in global memory,
+  // read from LDS by the off-block gather when it runs (volatile: not hoisted
+  // into registers held across the batch loop)
+  const SPIC_GLOBAL T *const volatile __attribute__((address_space(3))) *gptr;
   int c0x, c0y, c0z;
 };
 
@@ -149,9 +153,18 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
             }
       } else {
         ++n_glob;
-        X0 = gmem(g.xn)[ix - 1]; X1 = gmem(g.xn)[ix];
-        Y0 = gmem(g.yn)[iy - 1]; Y1 = gmem(g.yn)[iy];
-        Z0 = gmem(g.zn)[iz - 1]; Z1 = gmem(g.zn)[iz];
+        const SPIC_GLOBAL T *gp[9];
+        if constexpr (STAGED) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) gp[c] = blk.gptr[c];
+        } else {
+          gp[0] = gmem(f.Ex); gp[1] = gmem(f.Ey); gp[2] = gmem(f.Ez);
+          gp[3] = gmem(f.Bx); gp[4] = gmem(f.By); gp[5] = gmem(f.Bz);
+          gp[6] = gmem(g.xn); gp[7] = gmem(g.yn); gp[8] = gmem(g.zn);
+        }
+        X0 = gp[6][ix - 1]; X1 = gp[6][ix];
+        Y0 = gp[7][iy - 1]; Y1 = gp[7][iy];
+        Z0 = gp[8][iz - 1]; Z1 = gp[8][iz];
         const int gb = (ix * nyn + iy) * nzn + iz;
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa)
@@ -160,12 +173,8 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
               const int n = gb - aa * sxg - bb * nzn - cc;
-              F[aa][bb][cc][0] = gmem(f.Ex)[n];
-              F[aa][bb][cc][1] = gmem(f.Ey)[n];
-              F[aa][bb][cc][2] = gmem(f.Ez)[n];
-              F[aa][bb][cc][3] = gmem(f.Bx)[n];
-              F[aa][bb][cc][4] = gmem(f.By)[n];
-              F[aa][bb][cc][5] = gmem(f.Bz)[n];
+#pragma unroll
+              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = gp[c][n];
             }
       }
     }
@@ -422,6 +431,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
   __shared__ unsigned int s_nl;    // entries appended to the ring this species
   __shared__ unsigned int s_stay;  // particles written that stay in the tile
   __shared__ unsigned int s_base;  // start of the block reserved for them
+  __shared__ const SPIC_GLOBAL T *s_gptr[9];  // Block::gptr
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -432,6 +442,12 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
   blk.fld = s_fld;
   blk.xn = s_xn; blk.yn = s_yn; blk.zn = s_zn;
   blk.c0x = tx * TX; blk.c0y = ty * TY; blk.c0z = tz * TZ;
+  blk.gptr = (const SPIC_GLOBAL T *const volatile __attribute__((address_space(3))) *)s_gptr;
+  if (tid == 0) {
+    s_gptr[0] = gmem(a.f.Ex); s_gptr[1] = gmem(a.f.Ey); s_gptr[2] = gmem(a.f.Ez);
+    s_gptr[3] = gmem(a.f.Bx); s_gptr[4] = gmem(a.f.By); s_gptr[5] = gmem(a.f.Bz);
+    s_gptr[6] = gmem(a.g.xn); s_gptr[7] = gmem(a.g.yn); s_gptr[8] = gmem(a.g.zn);
+  }
   const int nyn = a.g.nyn, nzn = a.g.nzn;
 
   // 1. stage node coordinates and fields
