@@ -266,6 +266,53 @@ def test_engine_fast_particles_cross_tiles_every_phase():
             assert_close(got_m[s][n], st.moments[s][n], 1e-13, f"s{s} {n}")
 
 
+@pytest.mark.parametrize("strict", [True, False])
+@pytest.mark.parametrize("prec", PRECS)
+def test_engine_iterates_outside_the_staged_block(prec, strict):
+    """The ion species' particles in cell layer 3 of the gem_hot grid (8x4x4
+    cells, 4x4x4 tiles, so the last layer of tile 0) drift +4.4 cells per
+    step along x with the fixture's fields: the predictor-corrector iterates
+    (half a step, about 2.2 cells) land in cell layers 5-6, beyond the
+    staged block and its one-cell halo, so the push gathers the stencil
+    from global memory; the final positions reach tile 1 or wrap through
+    the periodic x face into tile 0.  Ions only: their gyration per step is
+    0.18 rad at the fixture's largest B, so the drift stays along x and the
+    iterates stay within the grid (the electrons' 11.6 rad would turn it
+    into y, towards the reflecting walls).  Strict arithmetic: trajectories
+    bit for bit; production arithmetic: within the bounds of the other
+    production tests."""
+    data = load_case("gem_hot", prec)
+    st = oracle_from_fixture("gem_hot", prec, data)
+    c = st.cfg
+    dx = c.Lx / c.nxc
+    ions = [s for s in range(c.ns) if c.qom[s] > 0]
+    assert len(ions) == 2
+    for s in ions:
+        p = st.parts[s]
+        layer3 = (p["x"] >= 3 * dx) & (p["x"] < 4 * dx)
+        assert int(np.sum(layer3)) == 32
+        p["u"][layer3] = 4.4 * dx / c.dt
+    S.set_strict_fp(strict)
+    got_p, got_m, stats = run_engine(st, prec, phases=1)
+    oracle_phases(st, 1)
+    assert stats["global_gathers"] >= 2 * 32, stats
+    for s in range(c.ns):
+        if strict:
+            for k in O.PARTICLE_NAMES:
+                assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+        else:
+            tol = 1e-12 if prec == "f64" else 1e-5
+            for k in ("u", "v", "w"):
+                assert_close(got_p[s][k], st.parts[s][k], tol, f"s{s} {k}")
+            for k in ("x", "y", "z"):
+                assert np.max(np.abs(got_p[s][k] - st.parts[s][k])) < (
+                    1e-11 if prec == "f64" else 1e-5), f"s{s} {k}"
+        for n in O.MOMENT_NAMES:
+            assert_close(got_m[s][n], st.moments[s][n],
+                         (1e-13 if strict else 1e-12) if prec == "f64" else 1e-5,
+                         f"s{s} {n}")
+
+
 @pytest.mark.parametrize("set_moments", [False, True])
 @pytest.mark.parametrize("prec", PRECS)
 def test_engine_overflow_region(prec, set_moments):
