@@ -80,12 +80,23 @@ __device__ __forceinline__ bool bclamp(int &ix, int n) {
   return bad;
 }
 
-// 1/(1 + omdtsq): in double (src/Mover.cpp:71) for FP64 and strict FP32;
-// the hardware reciprocal (1 ulp) in the FP32 form.
+// 1/(1 + omdtsq): the correctly rounded double division (src/Mover.cpp:71)
+// in strict arithmetic; in production FP64 the hardware reciprocal refined
+// by two Newton-Raphson steps r += r (1 - d r), within an ulp of the
+// quotient at half the instructions (d >= 1, so no special cases); the
+// hardware reciprocal (1 ulp) in the FP32 form.
 template <typename T>
 __device__ __forceinline__ T bdenom(T omdtsq) {
-  if (kStrictB || sizeof(T) == 8) return (T)(1.0 / (1.0 + (double)omdtsq));
-  return (T)__builtin_amdgcn_rcpf((float)(T(1) + omdtsq));
+  if (kStrictB) return (T)(1.0 / (1.0 + (double)omdtsq));
+  if constexpr (sizeof(T) == 8) {
+    const double d = 1.0 + omdtsq;
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    return r;
+  } else {
+    return (T)__builtin_amdgcn_rcpf((float)(T(1) + omdtsq));
+  }
 }
 
 // The staged block of a tile as the mover's stencil gather sees it: node
