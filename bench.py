@@ -59,7 +59,7 @@ def measured_traffic(prec: str):
 
 
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
-             tile, compare: bool = False) -> dict:
+             tile) -> dict:
     import ctypes as C
 
     import sputnipic_amd as S
@@ -144,17 +144,9 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
                 if timed:
                     marks.append(("kernel", a, b))
 
-    out_moments1 = None
     try:
         for w_ in range(warmup):
             cycle(False)
-            if w_ == 0 and compare:
-                # after one cycle from the initial condition, folded
-                stream.synchronize()
-                S.set_async(False)
-                S.applyBCids_all(moms, grd, param)
-                out_moments1 = [m.download() for m in moms]
-                S.set_async(True)
         stream.synchronize()
         check(L.spic_check_domain(stream.ptr), "domain check after warmup")
         state["ev"] = 0
@@ -215,23 +207,6 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         "sort_every": sort_every if eng is not None else None,
         "charge_rel_err": q_err,
     }
-    # state after the last cycle, kept (not printed) for the FP32-against-
-    # FP64 comparison: node moments after the ghost-node fold (applyBCids,
-    # which sums the periodic duplicate nodes, so a particle that wraps in
-    # one precision and not in the other compares equal), and x..w in load
-    # order
-    out_moments = out_parts = None
-    if compare:
-        S.applyBCids_all(moms, grd, param)
-        out_moments = [m.download() for m in moms]
-        if eng is not None:
-            out_parts = []
-            for s in range(NS):
-                dp = S.DeviceParticles(counts[s], 0.0, prec)
-                eng.store(s, dp)
-                out_parts.append({k: dp.d[k].download() for k in ("x", "y", "z", "u", "v", "w")})
-                for d in dp.d.values():
-                    d.free()
     if eng is not None:
         st = eng.stats()
         out["engine_device_bytes"] = st["device_bytes"]
@@ -244,9 +219,6 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         for p in parts:
             for d in p.d.values():
                 d.free()
-    out["_moments"] = out_moments
-    out["_moments1"] = out_moments1
-    out["_parts"] = out_parts
     return out
 
 
@@ -670,56 +642,6 @@ def run_field(reps: int, cpu: bool) -> dict:
     return out
 
 
-def form_errors(m32, m64, interior: bool = False) -> dict:
-    """FP32 form against FP64 form after the same cycles from the same
-    initial condition: per moment, max |m32 - m64| over all species and
-    nodes, relative to the largest |m64| of that moment.  With interior,
-    the three node planes next to each reflecting y wall are left out: a
-    particle within one rounding step of the wall can reflect in one
-    precision only, which flips its v and so the sign of its share of Jy,
-    pxy and pyz at those nodes."""
-    import numpy as np
-
-    nxn, nyn, nzn = GRID[0] + 3, GRID[1] + 3, GRID[2] + 3
-    keep = np.ones((nxn, nyn, nzn), bool)
-    if interior:
-        keep[:, :3, :] = False
-        keep[:, nyn - 3:, :] = False
-    keep = keep.reshape(-1)
-    err = {}
-    for n in m64[0]:
-        num = max(float(abs(a[n] - b[n])[keep].max()) for a, b in zip(m32, m64))
-        den = max(float(abs(b[n]).max()) for b in m64)
-        err[n] = num / den if den > 0 else num
-    return err
-
-
-def particle_errors(p32, p64) -> dict:
-    """Per particle component, FP32 form against FP64 form: positions as
-    |dx| over the box length (x and z modulo the periodic length), velocities
-    as |du| over the largest |u| of the FP64 form; the maximum and the root
-    mean square over all particles.  A particle within rounding of a
-    reflecting wall can reflect in one precision only, which flips its
-    velocity component: the maximum shows such particles, the RMS does not."""
-    import numpy as np
-
-    err = {}
-    scale = {"x": BOX[0], "y": BOX[1], "z": BOX[2]}
-    for k in ("u", "v", "w"):
-        scale[k] = max(float(np.abs(b[k]).max()) for b in p64)
-    for k in ("x", "y", "z", "u", "v", "w"):
-        worst, sq, cnt = 0.0, 0.0, 0
-        for a, b in zip(p32, p64):
-            d = np.abs(a[k].astype(np.float64) - b[k])
-            if k in ("x", "z"):
-                d = np.minimum(d, scale[k] - d)
-            worst = max(worst, float(d.max()))
-            sq += float(np.dot(d, d))
-            cnt += d.size
-        err[k] = {"max": worst / scale[k], "rms": (sq / max(cnt, 1)) ** 0.5 / scale[k]}
-    return err
-
-
 class _StdoutToStderr:
     """Route the process's C-level stdout to stderr, so that text the CPU
     baseline's compiled code prints (the reference's initGEM writes a banner
@@ -790,44 +712,209 @@ def host_cpu() -> dict:
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
-def cpu_baseline(cycles: int) -> dict:
-    """The reference's own CPU mover_PC + interpP2G (its source compiled at
-    the optimisation level of its Makefile, oracle/_ref/libref_f64_O3.so),
-    or the oracle restatement if that build is absent, timed on this host's
-    cores with the reference driver's loop structure, on a bounded sample
-    of the same workload: the 128x64x64 grid with 2x2x2 particles per cell
-    (16,777,216 particles)."""
-    import oracle as O
+REF_CYCLES = 10
 
+
+def moment_errors(got, want, shape, wall_planes: int = 3) -> dict:
+    """Per moment over the species: max |got - want| / max |want| over all
+    nodes, the same without the `wall_planes` node planes next to each
+    reflecting y wall, and ||got - want||_2 / ||want||_2 over all nodes."""
+    import numpy as np
+
+    out = {}
+    for n in want[0]:
+        num = num_in = den = l2n = l2d = 0.0
+        for g, w in zip(got, want):
+            d = np.abs(g[n] - w[n])
+            num = max(num, float(d.max()))
+            d3 = d.reshape(shape)[:, wall_planes:shape[1] - wall_planes, :]
+            num_in = max(num_in, float(d3.max()))
+            den = max(den, float(np.abs(w[n]).max()))
+            l2n += float(np.dot(d, d))
+            l2d += float(np.dot(w[n], w[n]))
+        out[n] = {"max_rel": num / den if den else num,
+                  "max_rel_interior": num_in / den if den else num_in,
+                  "l2_rel": (l2n / l2d) ** 0.5 if l2d else l2n ** 0.5}
+    return out
+
+
+class ParticleErrors:
+    """Per particle component, accumulated over species: max and RMS of
+    |got - want| over the box length (x and z modulo their period) or the
+    largest |want| of the velocity component, and the particles whose
+    velocity component has the other sign (reflected in one run only)."""
+
+    def __init__(self, vmax):
+        self.vmax = vmax
+        self.acc = {k: {"max": 0.0, "ss": 0.0, "n": 0, "flips": 0} for k in "xyzuvw"}
+
+    def add(self, got, want):
+        import numpy as np
+        for k in "xyzuvw":
+            g = got[k].astype(np.float64)
+            d = np.abs(g - want[k])
+            if k in "xyz":
+                L = BOX["xyz".index(k)]
+                if k != "y":
+                    d = np.minimum(d, L - d)
+                scale = L
+            else:
+                scale = self.vmax[k]
+                self.acc[k]["flips"] += int(np.count_nonzero(
+                    (np.sign(g) * np.sign(want[k]) < 0) & (d > 1e-3 * scale)))
+            a = self.acc[k]
+            a["max"] = max(a["max"], float(d.max()) / scale)
+            a["ss"] += float(np.dot(d, d)) / scale ** 2
+            a["n"] += d.size
+
+    def result(self):
+        out = {}
+        for k, a in self.acc.items():
+            out[k] = {"max": a["max"], "rms": (a["ss"] / max(a["n"], 1)) ** 0.5}
+            if k in "uvw":
+                out[k]["flips"] = a["flips"]
+        return out
+
+
+def reference_leg(cycles: int = REF_CYCLES) -> dict:
+    """The CPU reference on the benchmark's own workload, and the GPU forms
+    against it.
+
+    The 262,144,000-particle Harris sheet is generated on the device in FP64
+    (the same generator as the timed runs) and copied to the host, into the
+    reference's own state (oracle/_ref/libref_f64_O3.so: src/Mover.cpp etc.
+    built at the optimisation level of the reference's Makefile, FP64,
+    without initGEM; the oracle restatement where that build is absent).
+    One cycle of its mover_PC + zero + interpP2G with the reference
+    driver's OpenMP structure is the timed CPU baseline; it then runs to
+    `cycles` cycles.  The FP64 form and the FP32 form (fields rounded to
+    float; the FP64 particle arrays loaded into fixed-point positions and
+    float velocities and charges) run `cycles` engine phases on the device
+    from the same initial condition, are stored back into FP64 arrays and
+    compared with it after the ghost-node fold (applyBCids on both sides):
+    node moments per moment, particles per component in load order."""
+    import numpy as np
+
+    import oracle as O
+    import sputnipic_amd as S
+
+    S.set_async(False)
+    S.set_strict_fp(False)
+    grd64 = S.Grid(*GRID, *BOX, "f64")
+    gem = S.GemParams()
+    field64 = S.gem_fields(grd64, gem)
+    species = S.harris_species(PPC)
+    ic = [S.gem_particles(grd64, gem, sp, s, seed=0) for s, sp in enumerate(species)]
+    counts = [p.npmax for p in ic]
+    n = sum(counts)
     cfg = O.GemConfig(nxc=GRID[0], nyc=GRID[1], nzc=GRID[2], Lx=BOX[0], Ly=BOX[1],
-                      Lz=BOX[2], npcelx=[2] * NS, npcely=[2] * NS, npcelz=[2] * NS)
+                      Lz=BOX[2], npcelx=[PPC[0]] * NS, npcely=[PPC[1]] * NS,
+                      npcelz=[PPC[2]] * NS)
+    fields = {f: field64.d[f].download() for f in O.FIELD_NAMES}
     if O.ref_available("f64_O3"):
-        st = O.RefState(cfg, "f64_O3")
         kind = "reference"
-        n = sum(st.nop(s) for s in range(NS))
-        cores = st.max_threads()
+        ref = O.RefState(cfg, "f64_O3", ic=False)
+        for s, p in enumerate(ic):
+            assert ref.nop(s) == counts[s]
+            ref.set_particles(s, p.download())
+        for w, f in enumerate(O.FIELD_NAMES):
+            ref.set_field(w, fields[f])
+        cores = ref.max_threads()
+        timer = ref.time_cycles
     else:
-        st = O.State(cfg, "f64")
         kind = "port"
-        n = sum(len(p["x"]) for p in st.parts)
+        ref = O.State(cfg, "f64", build_ic=False)
+        for f in O.FIELD_NAMES:
+            ref.fields[f][:] = fields[f]
+        ref.parts = [p.download() for p in ic]
         cores = O.State.max_threads()
-    total, tm, td = st.time_cycles(cycles)
+        timer = ref.time_cycles
+    log(f"cpu {kind}: {n} particles on the host ({cores} threads)")
+    t1, tm1, td1 = timer(1)
+    log(f"cpu {kind}: cycle 1 in {t1:.2f} s (mover {tm1:.2f}, deposit {td1:.2f}): "
+        f"{n / t1 / 1e6:.1f} M particles/s")
+    t_rest = timer(cycles - 1)[0] if cycles > 1 else 0.0
+    log(f"cpu {kind}: cycles 2..{cycles} in {t_rest:.1f} s")
     if kind == "reference":
-        st.close()
-    value = n * cycles / total
-    log(f"cpu baseline ({kind}, {cores} threads): {value / 1e6:.1f} M particles/s "
-        f"(mover {tm:.2f} s, deposit {td:.2f} s over {cycles} cycles)")
-    return {
-        "value": value,
+        for s in range(NS):
+            ref.apply_bc_ids(s)
+        want_m = [{m: ref.moment(s, w) for w, m in enumerate(O.MOMENT_NAMES)}
+                  for s in range(NS)]
+        want_p = lambda s: ref.particles(s)  # noqa: E731
+    else:
+        for s in range(NS):
+            ref.apply_bc_ids(s)
+        want_m = ref.moments
+        want_p = lambda s: ref.parts[s]  # noqa: E731
+    vmax = {}
+    for s in range(NS):
+        p = want_p(s)
+        for k in "uvw":
+            vmax[k] = max(vmax.get(k, 0.0), float(np.abs(p[k]).max()))
+        del p
+    param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
+    errors = {}
+    for prec in ("f64", "f32"):
+        grd = S.Grid(*GRID, *BOX, prec)
+        if prec == "f64":
+            field = field64
+        else:
+            field = S.EMField(grd, {f: fields[f].astype(np.float32) for f in O.FIELD_NAMES})
+        eng = S.Engine(grd, NS, set_moments=True)
+        for s, p in enumerate(ic):
+            eng.load(s, p)  # the FP32 form takes the FP64 arrays (spic_engine_load_as)
+        moms = [S.InterpDensSpecies(grd) for _ in range(NS)]
+        for _ in range(cycles):
+            eng.phase(field, moms, param)
+        S.applyBCids_all(moms, grd, param)
+        got_m = [m.download() for m in moms]
+        me = moment_errors(got_m, want_m, (grd.nxn, grd.nyn, grd.nzn))
+        del got_m
+        pe = ParticleErrors(vmax)
+        for s in range(NS):
+            out = S.DeviceParticles(counts[s], species[s].qom, "f64")
+            eng.store(s, out)  # into FP64 arrays from either form
+            got = {k: out.d[k].download() for k in "xyzuvw"}
+            for d in out.d.values():
+                d.free()
+            pe.add(got, want_p(s))
+            del got
+        eng.close()
+        for m in moms:
+            for d in m.d.values():
+                d.free()
+        if prec == "f32":
+            for d in field.d.values():
+                d.free()
+        errors[prec] = {"moments": me, "particles": pe.result()}
+        log(f"{prec} against the {kind} after {cycles} cycles: moments max rel "
+            + ", ".join(f"{k} {v['max_rel']:.1e}" for k, v in me.items())
+            + "; interior " + ", ".join(f"{k} {v['max_rel_interior']:.1e}" for k, v in me.items())
+            + "; L2 " + ", ".join(f"{k} {v['l2_rel']:.1e}" for k, v in me.items()))
+        log(f"{prec} against the {kind}: particles (max, rms, flips) " + ", ".join(
+            f"{k} {v['max']:.1e} {v['rms']:.1e} {v.get('flips', '')}"
+            for k, v in errors[prec]["particles"].items()))
+    for p in ic:
+        for d in p.d.values():
+            d.free()
+    if kind == "reference":
+        ref.close()
+    cpu = {
+        "value": n / t1,
         "unit": "particles/s",
         "cores": cores,
         "kind": kind,
-        "sample": f"128x64x64 Harris sheet, 2x2x2 particles/cell x 4 species = {n} "
-                  f"particles, FP64, {cycles} cycles of mover_PC + zero + interpP2G "
-                  f"(OpenMP over particles in the mover, over species in the deposit)",
-        "seconds": total,
+        "sample": f"the benchmark's workload itself: 128x64x64 Harris sheet, 4 species x "
+                  f"5x5x5 per cell = {n} particles, FP64, one cycle of mover_PC + zero + "
+                  f"interpP2G (OpenMP over particles in the mover, over species in the "
+                  f"deposit), from the device generator's initial condition",
+        "seconds": t1,
+        "mover_seconds": tm1,
+        "deposit_seconds": td1,
         "host": host_cpu(),
     }
+    return {"cpu_baseline": cpu, "errors": errors, "cycles": cycles, "kind": kind,
+            "cpu_seconds_all_cycles": t1 + t_rest}
 
 
 def main() -> int:
@@ -836,7 +923,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=40)
+    ap.add_argument("--ref-cycles", type=int, default=REF_CYCLES,
+                    help="cycles of the CPU reference leg (the first one is the timed "
+                         "baseline; the GPU forms are compared with it after all of them)")
     ap.add_argument("--forms", default="f64,f32")
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
@@ -965,38 +1054,14 @@ def main() -> int:
     for prec in args.forms.split(","):
         tile = tuple(int(t) for t in args.tile.split("x"))
         forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, args.sort_every,
-                               tile, compare=len(args.forms.split(",")) > 1)
-    errors = errors1 = errorsi = errors1i = perrors = None
-    if "f64" in forms and "f32" in forms:
-        if forms["f64"]["_moments1"] is not None and forms["f32"]["_moments1"] is not None:
-            errors1 = form_errors(forms["f32"]["_moments1"], forms["f64"]["_moments1"])
-            errors1i = form_errors(forms["f32"]["_moments1"], forms["f64"]["_moments1"],
-                                   interior=True)
-            log("FP32 form against FP64 form after one cycle, max relative moment error: " +
-                ", ".join(f"{k} {v:.1e}" for k, v in errors1.items()))
-            log("  the same away from the reflecting walls: " +
-                ", ".join(f"{k} {v:.1e}" for k, v in errors1i.items()))
-        errors = form_errors(forms["f32"]["_moments"], forms["f64"]["_moments"])
-        errorsi = form_errors(forms["f32"]["_moments"], forms["f64"]["_moments"],
-                              interior=True)
-        log("FP32 form against FP64 form, max relative moment error: " +
-            ", ".join(f"{k} {v:.1e}" for k, v in errors.items()))
-        log("  the same away from the reflecting walls: " +
-            ", ".join(f"{k} {v:.1e}" for k, v in errorsi.items()))
-        if forms["f64"]["_parts"] is not None and forms["f32"]["_parts"] is not None:
-            perrors = particle_errors(forms["f32"]["_parts"], forms["f64"]["_parts"])
-            log("FP32 form against FP64 form, particle error (max, rms): " +
-                ", ".join(f"{k} {v['max']:.1e} {v['rms']:.1e}" for k, v in perrors.items()))
-    for f_ in forms.values():
-        f_.pop("_moments", None)
-        f_.pop("_moments1", None)
-        f_.pop("_parts", None)
+                               tile)
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
-    cpu = None
+    cpu = refleg = None
     if not args.no_cpu_baseline:
         with _StdoutToStderr():
-            cpu = cpu_baseline(args.cpu_cycles)
+            refleg = reference_leg(args.ref_cycles)
+        cpu = refleg["cpu_baseline"]
     order = None
     if not args.no_order and args.engine == "tiled":
         with _StdoutToStderr():
@@ -1050,22 +1115,24 @@ def main() -> int:
     }
     if "f64" in forms and "f32" in forms:
         line["fp32_over_fp64"] = forms["f32"]["value"] / forms["f64"]["value"]
-        line["fp32_error"] = {
-            "cycles": args.warmup + args.steps,
-            "moments_max_rel_err_after_1_cycle": errors1,
-            "moments_max_rel_err_after_1_cycle_interior": errors1i,
-            "moments_max_rel_err": errors,
-            "moments_max_rel_err_interior": errorsi,
-            "particles_rel_err": perrors,
-            "definition": "FP32 form against FP64 form after the same cycles from the same "
-                          "initial condition (FP32 values rounded from the FP64 "
-                          "generator's), frozen fields.  Moments: after the ghost-node fold "
-                          "(applyBCids), per moment max |FP32 - FP64| over species and nodes "
-                          "/ max |FP64|; interior: without the 3 node planes next to each reflecting "
-                          "y wall, where a particle within rounding of the wall reflects in one "
-                          "precision only.  Particles: max and rms over all particles of |dx| / box "
-                          "length (x, z modulo the period) and |du| / max |u|",
-        }
+    if refleg is not None:
+        definition = (
+            "against the reference's own mover_PC + interpP2G (FP64, CPU) after the same "
+            "cycles from the same initial condition (the device generator's, FP64; the "
+            "FP32 form gets it rounded to float), frozen fields, production arithmetic on "
+            "the GPU.  Moments after the ghost-node fold (applyBCids) on both sides: per "
+            "moment max |GPU - ref| over species and nodes / max |ref|, the same without "
+            "the 3 node planes next to each reflecting y wall (interior), and the "
+            "relative L2 norm over all nodes.  Particles in load order: max and rms of "
+            "|dx| / box length (x, z modulo the period) and |du| / max |u|; flips: "
+            "particles whose velocity component has the other sign (reflected at a wall "
+            "in one run only)")
+        for prec, key in (("f32", "fp32_error"), ("f64", "fp64_error")):
+            line[key] = {"reference": "oracle" if refleg["kind"] == "port" else
+                         "reference (oracle/_ref, the reference's own sources)",
+                         "cycles": refleg["cycles"], **refleg["errors"][prec],
+                         "definition": definition}
+        line["reference_leg_cpu_seconds"] = refleg["cpu_seconds_all_cycles"]
     print(json.dumps(line), flush=True)
     return 0
 

@@ -230,6 +230,7 @@ int spic_maxwell_image(const spic_grid *grd, const spic_field *field,
                        double *im, void *work, size_t work_bytes,
                        void *stream);
 
+
 /* ---------------------------------------------------------------------- */
 /* Tile-resident engine: the whole particle phase of a cycle -- mover_PC   */
 /* then interpP2G for every species, the loop of                          */
@@ -248,6 +249,8 @@ typedef struct spic_engine spic_engine;
 
 #define SPIC_SORT_TILE 0
 #define SPIC_SORT_RADIX 1
+#define SPIC_POS_FLOAT 1
+#define SPIC_POS_FIXED 2
 
 typedef struct spic_engine_opts {
   int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s) or 8x4x4 */
@@ -278,6 +281,17 @@ typedef struct spic_engine_opts {
                                  arriving from other tiles after the
                                  tile's own.  Trajectories do not depend
                                  on the mode; loads always radix sort */
+  int positions;              /* storage of positions: SPIC_POS_FLOAT the
+                                 reference's (FPpart lengths), SPIC_POS_FIXED
+                                 int32 fixed point in cell units (FP32
+                                 engines only: 2^-k cell resolution, k <= 24
+                                 chosen so the grid plus 64 cells fits, e.g.
+                                 2^-23 cells on 128 cells, against float's
+                                 2^-24 relative; exact wraps and
+                                 reflections).  0 (default): fixed point for
+                                 an FP32 engine created while strict
+                                 arithmetic is off, else float.  Strict
+                                 arithmetic refuses a fixed-point engine */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {
@@ -293,6 +307,7 @@ typedef struct spic_engine_stats {
   int64_t lds_gathers, global_gathers;
   int64_t lds_deposits, global_deposits;
   int64_t overflow;     /* particles now in overflow regions (all species) */
+  int positions;        /* SPIC_POS_FLOAT or SPIC_POS_FIXED */
 } spic_engine_stats;
 
 /* Create an engine for `ns` species (1..8) on `grd` (its node tables are
@@ -308,6 +323,16 @@ int spic_engine_load(spic_engine *eng, int is, const spic_species *sp,
  * (sp->nop must equal the loaded count). */
 int spic_engine_store(spic_engine *eng, int is, spic_species *sp,
                       void *stream);
+/* spic_engine_load / spic_engine_store with the caller's arrays in
+ * precision `prec` (SPIC_F64 or SPIC_F32), which may differ from the
+ * engine's: an FP32 engine loads FP64 arrays (positions converted from the
+ * FP64 values, so fixed-point storage keeps their precision; velocities and
+ * charges rounded to float) and stores into FP64 arrays.  An FP64 engine
+ * takes FP64 arrays only. */
+int spic_engine_load_as(spic_engine *eng, int is, const spic_species *sp,
+                        int prec, void *stream);
+int spic_engine_store_as(spic_engine *eng, int is, spic_species *sp, int prec,
+                         void *stream);
 /* One particle phase: for every species, mover_PC then interpP2G, ADDING
  * the moments into mom[is] (ns moment sets; zero them first as the
  * reference driver does, src/sputniPIC_CPU.cpp:247). */
@@ -430,6 +455,12 @@ int spic_set_strict_fp(int strict);
 /* In asynchronous mode: wait for `stream`, return SPIC_EDOMAIN if any
  * out-of-domain particle was seen since the last check, and clear it. */
 int spic_check_domain(void *stream);
+
+/* Testing hook (no counterpart in the reference): the engine device
+ * allocation `after` allocations from now (0: the next one) fails as an
+ * out-of-memory error would, without allocating; negative switches it off.
+ * Lets a test reach the engine's allocation-failure paths. */
+int spic_test_fail_engine_alloc(int after);
 
 int spic_device_count(int *count);
 int spic_malloc(void **dptr, size_t bytes);

@@ -496,11 +496,17 @@ class RefState:
 
     _libs: Dict[str, C.CDLL] = {}
 
-    def __init__(self, cfg: GemConfig, prec: str = "f64"):
+    def __init__(self, cfg: GemConfig, prec: str = "f64", ic: bool = True,
+                 io_dir: Optional[str] = None, restart: bool = False):
+        """ic False: skip initGEM; fields and particles (cfg's counts) are
+        then set with set_field / set_particles before use.  io_dir:
+        initGEM's IC dump directory -- it writes its save_ic_data files
+        there, or with restart reads read_ic_data's files from there
+        instead of computing the IC."""
         if cfg.ns > 6:
             raise ValueError("the reference holds at most NS_MAX = 6 species")
-        if (cfg.seed_base != 0 or cfg.pertX != 0.4 or cfg.fourpi != 12.5663706144
-                or cfg.th != 1.0):
+        if ic and (cfg.seed_base != 0 or cfg.pertX != 0.4 or cfg.fourpi != 12.5663706144
+                   or cfg.th != 1.0):
             raise ValueError("the reference IC is fixed to seed 0, pertX 0.4, default fourpi "
                              "and th")
         self.cfg, self.prec = cfg, prec
@@ -516,7 +522,10 @@ class RefState:
                       "u0", "v0", "w0"):
                 getattr(rc, k)[i] = getattr(cfg, k)[i]
         self.L = L
-        self.h = L.ref_create(C.byref(rc))
+        if io_dir is not None:
+            self.h = L.ref_create_io(C.byref(rc), io_dir.encode(), int(bool(restart)))
+        else:
+            self.h = (L.ref_create if ic else L.ref_create_empty)(C.byref(rc))
         ints = (C.c_int * 6)()
         dbl = (C.c_double * 13)()
         L.ref_grid(self.h, ints, dbl)
@@ -543,8 +552,12 @@ class RefState:
             if not os.path.exists(path):
                 raise RuntimeError(f"reference build missing: {path}")
             L = C.CDLL(path)
-            L.ref_create.restype = _P
-            L.ref_create.argtypes = [C.POINTER(RefCfg)]
+            for f in ("ref_create", "ref_create_empty"):
+                getattr(L, f).restype = _P
+                getattr(L, f).argtypes = [C.POINTER(RefCfg)]
+            L.ref_create_io.restype = _P
+            L.ref_create_io.argtypes = [C.POINTER(RefCfg), C.c_char_p, C.c_int]
+            L.ref_get_aux.argtypes = [_P, C.c_int, _P]
             L.ref_destroy.argtypes = [_P]
             L.ref_grid.argtypes = [_P, _P, _P]
             L.ref_nop.restype = C.c_long
@@ -604,6 +617,12 @@ class RefState:
     def field(self, which: int) -> np.ndarray:
         a = np.zeros(self.nn, np.float64)
         self.L.ref_get_field(self.h, which, _ptr(a))
+        return a
+
+    def aux(self, which: int) -> np.ndarray:
+        """0..2 Bxc Byc Bzc (cells), 3..5 Exth Eyth Ezth (nodes)."""
+        a = np.zeros(self.ncell if which < 3 else self.nn, np.float64)
+        self.L.ref_get_aux(self.h, which, _ptr(a))
         return a
 
     def set_field(self, which: int, arr: np.ndarray):

@@ -94,10 +94,22 @@ static void set_defaults(parameters *p) {
   p->SaveDirName = p->RestartDirName;
 }
 
-void *ref_create(const ref_cfg *cfg) {
+// The reference's state for cfg; with `ic`, initGEM fills fields and
+// particles, else they stay as the allocators leave them, for the caller to
+// set (ref_set_field / ref_set_particles).  With io_dir, initGEM's own
+// dump directory: it writes its save_ic_data files there (restart 0) or
+// reads its read_ic_data files from there instead of computing the IC
+// (restart 1).
+static void *create_state(const ref_cfg *cfg, bool ic, const char *io_dir = nullptr,
+                          int restart = 0) {
   ref_state *s = new ref_state();
   parameters *p = &s->param;
   set_defaults(p);
+  if (io_dir) {
+    p->RestartDirName = io_dir;
+    p->SaveDirName = io_dir;
+    p->RESTART = restart != 0;
+  }
   p->nxc = cfg->nxc;
   p->nyc = cfg->nyc;
   p->nzc = cfg->nzc;
@@ -147,9 +159,38 @@ void *ref_create(const ref_cfg *cfg) {
   long nc = (long)s->grd.nxc * s->grd.nyc * s->grd.nzc;
   for (int is = 0; is < p->ns; is++)
     std::memset(s->ids[is].rhoc_flat, 0, sizeof(FPinterp) * nc);
-  initGEM(p, &s->grd, &s->field, &s->field_aux, s->part, s->ids);
+  // the cell-centred B is computed on interior cells only: defined guards
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  std::memset(s->field_aux.Bxc_flat, 0, sizeof(FPfield) * nc);
+  std::memset(s->field_aux.Byc_flat, 0, sizeof(FPfield) * nc);
+  std::memset(s->field_aux.Bzc_flat, 0, sizeof(FPfield) * nc);
+  std::memset(s->field_aux.Exth_flat, 0, sizeof(FPfield) * nn);
+  std::memset(s->field_aux.Eyth_flat, 0, sizeof(FPfield) * nn);
+  std::memset(s->field_aux.Ezth_flat, 0, sizeof(FPfield) * nn);
+  if (ic) initGEM(p, &s->grd, &s->field, &s->field_aux, s->part, s->ids);
   return s;
 }
+
+void *ref_create(const ref_cfg *cfg) { return create_state(cfg, true); }
+
+// initGEM with its IC dump directory (save_ic_data or read_ic_data).
+void *ref_create_io(const ref_cfg *cfg, const char *dir, int restart) {
+  return create_state(cfg, true, dir, restart);
+}
+
+// which: 0..2 = Bxc Byc Bzc (cells), 3..5 = Exth Eyth Ezth (nodes)
+void ref_get_aux(void *h, int which, double *out) {
+  ref_state *s = (ref_state *)h;
+  long nn = (long)s->grd.nxn * s->grd.nyn * s->grd.nzn;
+  long nc = (long)s->grd.nxc * s->grd.nyc * s->grd.nzc;
+  FPfield *a[6] = {s->field_aux.Bxc_flat,  s->field_aux.Byc_flat,  s->field_aux.Bzc_flat,
+                   s->field_aux.Exth_flat, s->field_aux.Eyth_flat, s->field_aux.Ezth_flat};
+  to_double(a[which], out, which < 3 ? nc : nn);
+}
+
+// Without initGEM: for inputs made elsewhere (the device generator's
+// Harris sheet at the benchmark's size), copied in afterwards.
+void *ref_create_empty(const ref_cfg *cfg) { return create_state(cfg, false); }
 
 void ref_destroy(void *h) {
   ref_state *s = (ref_state *)h;

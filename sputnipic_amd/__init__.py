@@ -7,7 +7,7 @@ The compute lives in ``lib/libspic.so`` (hand-written HIP, C ABI in
 reference's interface over that ABI.  There is no CPU fallback.
 """
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
-from ._capi import SPIC_SORT_RADIX, SPIC_SORT_TILE  # noqa: F401
+from ._capi import SPIC_POS_FIXED, SPIC_POS_FLOAT, SPIC_SORT_RADIX, SPIC_SORT_TILE  # noqa: F401
 from .core import (Batch, DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams, Graph,  # noqa: F401
                    InterpDensAux, MaxwellImage, SolveParams, applyBCscalar, calculateHatDensities,
                    interpN2Crho_net,

@@ -23,6 +23,8 @@ SPIC_F64 = 0
 SPIC_F32 = 1
 SPIC_SORT_TILE = 0
 SPIC_SORT_RADIX = 1
+SPIC_POS_FLOAT = 1
+SPIC_POS_FIXED = 2
 
 STATUS_NAMES = {
     SPIC_OK: "SPIC_OK", SPIC_EINVAL: "SPIC_EINVAL", SPIC_EHIP: "SPIC_EHIP",
@@ -95,7 +97,8 @@ class spic_gem_species(C.Structure):
 class spic_engine_opts(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
                 ("sort_every", C.c_int), ("lds_partials", C.c_int),
-                ("slack_pct", C.c_int), ("set_moments", C.c_int), ("sort_mode", C.c_int)]
+                ("slack_pct", C.c_int), ("set_moments", C.c_int), ("sort_mode", C.c_int),
+                ("positions", C.c_int)]
 
 
 class spic_engine_stats(C.Structure):
@@ -104,7 +107,7 @@ class spic_engine_stats(C.Structure):
                 ("device_bytes", C.c_size_t),
                 ("lds_gathers", C.c_int64), ("global_gathers", C.c_int64),
                 ("lds_deposits", C.c_int64), ("global_deposits", C.c_int64),
-                ("overflow", C.c_int64)]
+                ("overflow", C.c_int64), ("positions", C.c_int)]
 
 
 class spic_batch_member(C.Structure):
@@ -146,6 +149,8 @@ SIGNATURES = {
     "spic_engine_destroy": (C.c_int, [_P]),
     "spic_engine_load": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
     "spic_engine_store": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), _P]),
+    "spic_engine_load_as": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), C.c_int, _P]),
+    "spic_engine_store_as": (C.c_int, [_P, C.c_int, C.POINTER(spic_species), C.c_int, _P]),
     "spic_engine_deposit": (C.c_int, [_P, C.POINTER(spic_moments), _P]),
     "spic_engine_phase": (C.c_int, [_P, C.POINTER(spic_field), C.POINTER(spic_moments),
                                     C.POINTER(spic_push), _P]),
@@ -182,6 +187,7 @@ SIGNATURES = {
     "spic_set_async": (C.c_int, [C.c_int]),
     "spic_set_strict_fp": (C.c_int, [C.c_int]),
     "spic_check_domain": (C.c_int, [_P]),
+    "spic_test_fail_engine_alloc": (C.c_int, [C.c_int]),
     "spic_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "spic_malloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
     "spic_free": (C.c_int, [_P]),

@@ -614,27 +614,35 @@ class Engine:
 
     def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
                  lds_partials: int = 0, slack_pct: int = 0, set_moments: bool = False,
-                 sort_mode: int = 0):
+                 sort_mode: int = 0, positions: int = 0):
+        """positions: 0 (default: fixed point for an FP32 engine created in
+        production arithmetic, else float), SPIC_POS_FLOAT or
+        SPIC_POS_FIXED."""
         from ._capi import spic_engine_opts
         self.grd = grd
         self.ns = ns
         p = C.c_void_p()
         opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials,
-                                slack_pct, int(bool(set_moments)), int(sort_mode))
+                                slack_pct, int(bool(set_moments)), int(sort_mode),
+                                int(positions))
         g = grd.c_struct()
         check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
               "spic_engine_create")
         self.ptr = p.value
 
     def load(self, is_: int, part: Particles, stream: Optional["Stream"] = None) -> None:
+        """Load species is_ from `part`, whose precision may be FP64 for an
+        FP32 engine (spic_engine_load_as)."""
         s = part.c_struct()
-        check(lib().spic_engine_load(self.ptr, is_, C.byref(s), stream.ptr if stream else None),
-              "spic_engine_load")
+        check(lib().spic_engine_load_as(self.ptr, is_, C.byref(s), _prec_code(part.prec),
+                                        stream.ptr if stream else None), "spic_engine_load")
 
     def store(self, is_: int, part: Particles, stream: Optional["Stream"] = None) -> None:
+        """Store species is_ into `part` in load order, in part's precision
+        (spic_engine_store_as)."""
         s = part.c_struct()
-        check(lib().spic_engine_store(self.ptr, is_, C.byref(s),
-                                      stream.ptr if stream else None), "spic_engine_store")
+        check(lib().spic_engine_store_as(self.ptr, is_, C.byref(s), _prec_code(part.prec),
+                                         stream.ptr if stream else None), "spic_engine_store")
 
     def phase(self, field: EMField, ids: Sequence[InterpDensSpecies], param: Parameters,
               stream: Optional["Stream"] = None) -> None:

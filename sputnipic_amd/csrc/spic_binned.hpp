@@ -81,6 +81,23 @@ __host__ __device__ inline RangeView range_view(PS seg, PC cnt, int ntiles,
   return v;
 }
 
+// Fixed-point particle positions (FP32 engines in production arithmetic):
+// the x y z words of a slot hold int32 coordinates in units of 2^-k[a]
+// cells from the grid's start, X = (x - xStart) / dx * 2^k, so that the
+// cell is X >> k and the offset within it (X - (cell << k)) * 2^-k, exact;
+// k is chosen per axis so that every cell of the grid plus 64 cells of
+// excursion fit in int32 (k <= 24, so offsets convert to float exactly).
+// Wraps and reflections at the box faces are exact integer operations.
+struct FixedParams {
+  int k[3];
+  int lim[3];     // box length per axis: nc << k
+  float unit[3];  // 2^-k
+  float s[3];     // 2^k / dx: fixed units per unit length
+  double to_fixed[3];    // the same in double, for conversions from caller arrays
+  double from_fixed[3];  // dx * 2^-k, back to lengths
+  double start[3];       // xStart yStart zStart
+};
+
 // Storage type of the tile partials: the particle precision.  An FP32
 // engine sums its LDS block in FP64 by default and rounds each tile's block
 // to float once; the node reduction adds the tiles' floats in FP64.
@@ -114,6 +131,8 @@ struct BinnedArgs {
   int lds_fp64;            // FP32 particles: LDS moment block in FP64
   int n_ovf_blocks;        // workgroups for the overflow regions
   int push_off;            // 1: deposit only (interpP2G); no push, no field read
+  int fixed;               // 1: positions in fixed point (FP32 engines), see above
+  FixedParams fx;
   BinnedSpecies<T> sp[kMaxSpecies];
   // status words: [0] mover iterate out of the grid, [1] deposit cell out
   // of the grid, [2] overflow region full
@@ -158,7 +177,8 @@ struct CountArrays {
   hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab,           \
                                        const BatchMap &tiles,              \
                                        const BatchMap &ovf, int tx, int ty,\
-                                       int tz, int lds_fp64, hipStream_t s);\
+                                       int tz, int lds_fp64, int fixed,    \
+                                       hipStream_t s);                     \
   }
 SPIC_DECLARE_BINNED_LAUNCHERS(spic_fast)
 SPIC_DECLARE_BINNED_LAUNCHERS(spic_strict)
@@ -178,7 +198,31 @@ struct BinLayout {
   int ntiles;
   int tile_bits, rank_bits, cell_bits;
   int64_t slack;      // free slots per segment after a sort
+  int fixed;          // binned positions are fixed point (FixedParams fx)
+  FixedParams fx;
 };
+
+// Fixed-point parameters of a grid: per axis the largest k <= 24 with
+// (cells + 64) << k below 2^31.
+inline FixedParams fixed_params(int ncx, int ncy, int ncz, double dx, double dy, double dz,
+                                double xs, double ys, double zs) {
+  FixedParams f;
+  const int nc[3] = {ncx, ncy, ncz};
+  const double d[3] = {dx, dy, dz}, st[3] = {xs, ys, zs};
+  for (int a = 0; a < 3; ++a) {
+    int k = 24;
+    while (k > 0 && ((int64_t)nc[a] + 64) << k > (int64_t)0x7fffffff) --k;
+    f.k[a] = k;
+    f.lim[a] = nc[a] << k;
+    const double sc = (double)(1 << k);
+    f.unit[a] = (float)(1.0 / sc);
+    f.to_fixed[a] = sc / d[a];
+    f.s[a] = (float)f.to_fixed[a];
+    f.from_fixed[a] = d[a] / sc;
+    f.start[a] = st[a];
+  }
+  return f;
+}
 
 // Sums the tile partials of every species into its node moments (+=).
 hipError_t launch_binned_reduce(const BinnedReduceArgs &a, int tx, int ty, int tz,
@@ -200,25 +244,30 @@ hipError_t launch_batch_zero_counts(const BinnedArgs<T> *tab, const BatchMap &co
                                     int max_ns, int moments, hipStream_t s);
 
 // Sort of n particles of one species into binned storage (spic_binsort.hip).
-// The source is dense (src_seg == nullptr: particles 0..n-1, ids assigned
-// as their index) or binned (src_seg, src_cnt, src_ovf_cap; ids from
-// src_id).  The destination gets segments of count + slack slots:
+// The source is dense (src_seg == nullptr: particles 0..n-1 of the caller's
+// arrays, FP64 when src_f64 (an FP32 engine rounds them on the way in),
+// ids assigned as their index; with fixed-point storage the positions are
+// converted) or binned (the engine's own storage: src_seg, src_cnt,
+// src_ovf_cap; ids from src_id).  The destination gets segments of count +
+// slack slots:
 // dst_seg[t] = (particles in tiles < t) + t * slack, dst_cnt the counts
 // (no unused runs, an empty overflow region).  With workspace == nullptr
 // only *needed is set.
 template <typename T>
 hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
-                    T *const src[7], const uint32_t *src_id,
+                    const void *const src[7], int src_f64, const uint32_t *src_id,
                     const int64_t *src_seg, const unsigned int *src_cnt,
                     unsigned int src_ovf_cap, T *const dst[7], uint32_t *dst_id,
                     int64_t *dst_seg, unsigned int *dst_cnt, void *workspace,
                     size_t workspace_bytes, size_t *needed, hipStream_t s);
 
-// dst[id[i]] = src[i] for every particle in binned storage, seven arrays.
+// dst[id[i]] = src[i] for every particle in binned storage, seven arrays,
+// into FP64 arrays (dst_f64) or FP32 ones (FP32 engines only); fixed-point
+// positions are converted back to lengths.
 template <typename T>
 hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
-                     unsigned int ovf_cap, T *const dst[7], hipStream_t s);
+                     unsigned int ovf_cap, void *const dst[7], int dst_f64, hipStream_t s);
 
 // Tile sort of one binned species (a job) from buffer src (layout
 // src_seg) to buffer dst (layout dst_seg): the binned source's alternative

@@ -99,6 +99,29 @@ __device__ __forceinline__ T bdenom(T omdtsq) {
   }
 }
 
+// Fixed-point positions (FixedParams, FP32 engines in production
+// arithmetic).  The mover iterate's cell truncates the cell coordinate
+// toward zero as the reference's int() does (src/Mover.cpp:45-47); the
+// deposit cell is its floor (src/Mover.cpp:411-413).
+__device__ __forceinline__ int fx_trunc_cell(int X, int k) {
+  return X >= 0 ? (X >> k) : -((-X) >> k);
+}
+
+// A displacement d (fixed units) rounded to an integer, bounded to 32
+// cells: with k chosen so that (cells + 64) << k fits in int32, the sum
+// with a coordinate in the box plus its excursion cannot overflow; a
+// particle that moves farther leaves the grid and is reported.
+__device__ __forceinline__ int fx_step(float d, int k) {
+  const float lim = (float)(32 << k);
+  return __float2int_rn(fminf(fmaxf(d, -lim), lim));
+}
+
+// Offset of X from the left node of cell c, in cells: exact while X lies
+// in the cell (k <= 24 bits), extrapolated beyond it.
+__device__ __forceinline__ float fx_frac(int X, int c, int k, float unit) {
+  return (float)(X - (c << k)) * unit;
+}
+
 // The staged block of a tile as the mover's stencil gather sees it: node
 // fields [NB][6] and node coordinates in LDS, and the global node index of
 // block node 0 along each axis.
@@ -265,6 +288,147 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
   }
 }
 
+// push_particle for fixed-point positions (FP32 form): the same
+// predictor-corrector iteration, rotation and final update, with the
+// interpolation weights as offsets within the cell in cell units (so that
+// the weight is xi eta zeta invVOL of the reference without the multiply),
+// the iterate and the new position advanced by displacements rounded to the
+// fixed-point unit, and the boundary conditions applied exactly on the
+// integers: x > Lx is X > lim, a reflection lim - (X - lim), a wrap
+// X - lim.
+template <bool STAGED, int TX, int TY, int TZ>
+__device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<float> &f,
+                                           const KPush<float> &k, const FixedParams &fp,
+                                           const Block<float, TX, TY, TZ> &blk, int &X,
+                                           int &Y, int &Z, float &u, float &v, float &w,
+                                           bool &bad, unsigned int &n_lds,
+                                           unsigned int &n_glob) {
+  using B = Block<float, TX, TY, TZ>;
+  const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
+  const int Xt = X, Yt = Y, Zt = Z;
+  int Xc = X, Yc = Y, Zc = Z;
+  float upt = 0.f, vpt = 0.f, wpt = 0.f;
+  int cix = -100, ciy = -100, ciz = -100;  // cell of the cached stencil
+  float F[2][2][2][6];
+  // fixed units per unit velocity over half a step
+  const float hx = k.dto2 * fp.s[0], hy = k.dto2 * fp.s[1], hz = k.dto2 * fp.s[2];
+  for (int it = 0; it < k.niter; ++it) {
+    int ix = 2 + fx_trunc_cell(Xc, fp.k[0]);
+    int iy = 2 + fx_trunc_cell(Yc, fp.k[1]);
+    int iz = 2 + fx_trunc_cell(Zc, fp.k[2]);
+    bad |= bclamp(ix, g.nxn);
+    bad |= bclamp(iy, g.nyn);
+    bad |= bclamp(iz, g.nzn);
+    if (ix != cix || iy != ciy || iz != ciz) {
+      cix = ix; ciy = iy; ciz = iz;
+      const int bx = ix - 1 - blk.c0x, by = iy - 1 - blk.c0y, bz = iz - 1 - blk.c0z;
+      const bool in_block = STAGED && (bx >= 0) & (bx <= B::BX - 2) & (by >= 0) &
+                                          (by <= B::BY - 2) & (bz >= 0) & (bz <= B::BZ - 2);
+      if (in_block) {
+        ++n_lds;
+        const float *fb = blk.fld + (((bx + 1) * B::BY + (by + 1)) * B::BZ + (bz + 1)) * 6;
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              const int off = ((aa * B::BY + bb) * B::BZ + cc) * 6;
+#pragma unroll
+              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = fb[c - off];
+            }
+      } else {
+        ++n_glob;
+        const SPIC_GLOBAL float *gp[6];
+        if constexpr (STAGED) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gp[c] = blk.gptr[c];
+        } else {
+          gp[0] = gmem(f.Ex); gp[1] = gmem(f.Ey); gp[2] = gmem(f.Ez);
+          gp[3] = gmem(f.Bx); gp[4] = gmem(f.By); gp[5] = gmem(f.Bz);
+        }
+        const int gb = (ix * nyn + iy) * nzn + iz;
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              const int n = gb - aa * sxg - bb * nzn - cc;
+#pragma unroll
+              for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = gp[c][n];
+            }
+      }
+    }
+    const float ox = fx_frac(Xc, ix - 2, fp.k[0], fp.unit[0]);
+    const float oy = fx_frac(Yc, iy - 2, fp.k[1], fp.unit[1]);
+    const float oz = fx_frac(Zc, iz - 2, fp.k[2], fp.unit[2]);
+    const float xi[2] = {ox, 1.f - ox};
+    const float eta[2] = {oy, 1.f - oy};
+    const float zeta[2] = {oz, 1.f - oz};
+    float Exl = 0.f, Eyl = 0.f, Ezl = 0.f, Bxl = 0.f, Byl = 0.f, Bzl = 0.f;
+#pragma unroll
+    for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+          const float wgt = xi[aa] * eta[bb] * zeta[cc];
+          Exl += wgt * F[aa][bb][cc][0];
+          Eyl += wgt * F[aa][bb][cc][1];
+          Ezl += wgt * F[aa][bb][cc][2];
+          Bxl += wgt * F[aa][bb][cc][3];
+          Byl += wgt * F[aa][bb][cc][4];
+          Bzl += wgt * F[aa][bb][cc][5];
+        }
+    const float q2 = k.qomdt2;
+    const float omdtsq = q2 * q2 * (Bxl * Bxl + Byl * Byl + Bzl * Bzl);
+    const float denom = bdenom<float>(omdtsq);
+    const float ut = u + q2 * Exl;
+    const float vt = v + q2 * Eyl;
+    const float wt = w + q2 * Ezl;
+    const float udotb = ut * Bxl + vt * Byl + wt * Bzl;
+    upt = (ut + q2 * (vt * Bzl - wt * Byl + q2 * udotb * Bxl)) * denom;
+    vpt = (vt + q2 * (wt * Bxl - ut * Bzl + q2 * udotb * Byl)) * denom;
+    wpt = (wt + q2 * (ut * Byl - vt * Bxl + q2 * udotb * Bzl)) * denom;
+    Xc = Xt + fx_step(upt * hx, fp.k[0]);
+    Yc = Yt + fx_step(vpt * hy, fp.k[1]);
+    Zc = Zt + fx_step(wpt * hz, fp.k[2]);
+  }
+  u = 2.f * upt - u;
+  v = 2.f * vpt - v;
+  w = 2.f * wpt - w;
+  X = Xt + fx_step(upt * (k.dt_sub * fp.s[0]), fp.k[0]);
+  Y = Yt + fx_step(vpt * (k.dt_sub * fp.s[1]), fp.k[1]);
+  Z = Zt + fx_step(wpt * (k.dt_sub * fp.s[2]), fp.k[2]);
+  // particle boundary conditions, two independent tests per axis
+  // (src/Mover.cpp:104-150), exact on the integers
+  if (X > fp.lim[0]) {
+    if (k.perx) X -= fp.lim[0];
+    else { u = -u; X = fp.lim[0] - (X - fp.lim[0]); }
+  }
+  if (X < 0) {
+    if (k.perx) X += fp.lim[0];
+    else { u = -u; X = -X; }
+  }
+  if (Y > fp.lim[1]) {
+    if (k.pery) Y -= fp.lim[1];
+    else { v = -v; Y = fp.lim[1] - (Y - fp.lim[1]); }
+  }
+  if (Y < 0) {
+    if (k.pery) Y += fp.lim[1];
+    else { v = -v; Y = -Y; }
+  }
+  if (Z > fp.lim[2]) {
+    if (k.perz) Z -= fp.lim[2];
+    else { w = -w; Z = fp.lim[2] - (Z - fp.lim[2]); }
+  }
+  if (Z < 0) {
+    if (k.perz) Z += fp.lim[2];
+    else { w = -w; Z = -Z; }
+  }
+}
+
 // Tile of the deposit cell (ix, iy, iz), clamped to the physical cells;
 // the tile shape is a compile-time constant (divisions become shifts).
 template <int TX, int TY, int TZ, typename T>
@@ -417,11 +581,93 @@ __device__ __forceinline__ bool deposit_node(const KGrid<T> &g, const KMom &m, A
   return in_block;
 }
 
+// deposit_node and deposit_global for fixed-point positions: the weight of
+// corner (aa, bb, cc) is q times the offsets of the particle within its
+// deposit cell (ix, iy, iz) in cell units, q xi eta zeta invVOL of the
+// reference; each moment term is then that weight times p_m times invVOL,
+// as the reference adds it (src/Mover.cpp:430-568).
+template <typename A, int BX, int BY, int BZ>
+__device__ __forceinline__ bool deposit_node_fx(const KGrid<float> &g, const FixedParams &fp,
+                                                const KMom &m, A *s_mom, int c0x, int c0y,
+                                                int c0z, int ix, int iy, int iz, int aa, int bb,
+                                                int cc, int X, int Y, int Z, float u, float v,
+                                                float w, float q) {
+  constexpr int NB = BX * BY * BZ;
+  const int bx = ix - 1 - c0x, by = iy - 1 - c0y, bz = iz - 1 - c0z;
+  const bool in_block = (bx >= 0) & (bx <= BX - 2) & (by >= 0) & (by <= BY - 2) &
+                        (bz >= 0) & (bz <= BZ - 2);
+  const float ox = fx_frac(X, ix - 2, fp.k[0], fp.unit[0]);
+  const float oy = fx_frac(Y, iy - 2, fp.k[1], fp.unit[1]);
+  const float oz = fx_frac(Z, iz - 2, fp.k[2], fp.unit[2]);
+  const float xi = aa == 0 ? ox : 1.f - ox;
+  const float eta = bb == 0 ? oy : 1.f - oy;
+  const float zeta = cc == 0 ? oz : 1.f - oz;
+  const float iv = g.invVOL;
+  const float wgt = q * xi * eta * zeta;
+  A t[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) t[k] = A(0);
+  add_terms<float, A>(t, wgt, iv, u, v, w, u * u, u * v, u * w, v * v, v * w, w * w);
+  if (in_block) {
+    const int nb = ((bx + 1 - aa) * BY + (by + 1 - bb)) * BZ + (bz + 1 - cc);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) atomicAdd(&s_mom[k * NB + nb], t[k]);
+  } else {
+    const int n = ((ix - aa) * g.nyn + (iy - bb)) * g.nzn + (iz - cc);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) gatomic_add(m.m[k] + n, (double)t[k]);
+  }
+  return in_block;
+}
+
+__device__ __forceinline__ void deposit_global_fx(const KGrid<float> &g, const FixedParams &fp,
+                                                  const KMom &m, int ix, int iy, int iz, int X,
+                                                  int Y, int Z, float u, float v, float w,
+                                                  float q) {
+  const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
+  const float ox = fx_frac(X, ix - 2, fp.k[0], fp.unit[0]);
+  const float oy = fx_frac(Y, iy - 2, fp.k[1], fp.unit[1]);
+  const float oz = fx_frac(Z, iz - 2, fp.k[2], fp.unit[2]);
+  const float xi[2] = {ox, 1.f - ox};
+  const float eta[2] = {oy, 1.f - oy};
+  const float zeta[2] = {oz, 1.f - oz};
+  const float iv = g.invVOL;
+  const float uu = u * u, uv = u * v, uw = u * w, vv = v * v, vw = v * w, ww = w * w;
+  const int gb = (ix * nyn + iy) * nzn + iz;
+#pragma unroll
+  for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const float wgt = q * xi[aa] * eta[bb] * zeta[cc];
+        double t[10];
+#pragma unroll
+        for (int mm = 0; mm < 10; ++mm) t[mm] = 0.0;
+        add_terms<float, double>(t, wgt, iv, u, v, w, uu, uv, uw, vv, vw, ww);
+        const int n = gb - aa * sxg - bb * nzn - cc;
+#pragma unroll
+        for (int mm = 0; mm < 10; ++mm) gatomic_add(m.m[mm] + n, t[mm]);
+      }
+}
+
+// Deposit cell of a fixed-point position, 2 + floor of the cell
+// coordinate, clamped as bclamp does.
+__device__ __forceinline__ int fx_deposit_cell(int X, int k, int n, bool &bad) {
+  int c = 2 + (X >> k);
+  bad |= bclamp(c, n);
+  return c;
+}
+
 // A: type of the LDS moment block (T, or double for FP32 particles); the
 // per-thread register sums are in the particle precision T.  FP32 kernels
-// ask for three waves per SIMD (at most 168 VGPRs).
-template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true>
+// ask for three waves per SIMD (at most 168 VGPRs).  FX: fixed-point
+// positions (FP32, production arithmetic): the x y z words of the particle
+// storage, the output buffer and the list ring hold the int32 coordinates,
+// the buckets the particle's offsets within its deposit cell.
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const int tile) {
+  static_assert(!FX || (sizeof(T) == 4 && !kStrictB), "fixed point: FP32 production only");
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
   constexpr int NT = 4 * NC;        // threads: four per cell
@@ -547,15 +793,28 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       if (PREFETCH && j + 1 < nbatches) load_particle(r + NT);
       int ix = 2, iy = 2, iz = 2, dest = -1;
       if (active) {
-        if constexpr (PUSH)
-          push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
-                                             bad_move, n_lg, n_gg);
-        ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
-        iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
-        iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
-        bad_dep |= bclamp(ix, a.g.nxn);
-        bad_dep |= bclamp(iy, a.g.nyn);
-        bad_dep |= bclamp(iz, a.g.nzn);
+        if constexpr (FX) {
+          int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
+          if constexpr (PUSH)
+            push_fixed<true, TX, TY, TZ>(a.g, a.f, sp.k, a.fx, blk, X, Y, Z, u, v, w,
+                                         bad_move, n_lg, n_gg);
+          ix = fx_deposit_cell(X, a.fx.k[0], a.g.nxn, bad_dep);
+          iy = fx_deposit_cell(Y, a.fx.k[1], a.g.nyn, bad_dep);
+          iz = fx_deposit_cell(Z, a.fx.k[2], a.g.nzn, bad_dep);
+          x = __int_as_float(X);
+          y = __int_as_float(Y);
+          z = __int_as_float(Z);
+        } else {
+          if constexpr (PUSH)
+            push_particle<T, true, TX, TY, TZ>(a.g, a.f, sp.k, blk, x, y, z, u, v, w,
+                                               bad_move, n_lg, n_gg);
+          ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
+          iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
+          iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
+          bad_dep |= bclamp(ix, a.g.nxn);
+          bad_dep |= bclamp(iy, a.g.nyn);
+          bad_dep |= bclamp(iz, a.g.nzn);
+        }
         dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
       }
 
@@ -572,7 +831,14 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           const unsigned int sl = atomicAdd(&s_cnt[buf][cell], 1u);
           if (sl < (unsigned int)CAP) {
             const int e = cell * CS + (int)sl;
-            s_bkt[0][e] = x; s_bkt[1][e] = y; s_bkt[2][e] = z; s_bkt[3][e] = q;
+            if constexpr (FX) {
+              s_bkt[0][e] = fx_frac(__float_as_int(x), ix - 2, a.fx.k[0], a.fx.unit[0]);
+              s_bkt[1][e] = fx_frac(__float_as_int(y), iy - 2, a.fx.k[1], a.fx.unit[1]);
+              s_bkt[2][e] = fx_frac(__float_as_int(z), iz - 2, a.fx.k[2], a.fx.unit[2]);
+            } else {
+              s_bkt[0][e] = x; s_bkt[1][e] = y; s_bkt[2][e] = z;
+            }
+            s_bkt[3][e] = q;
             s_bkt[4][e] = u; s_bkt[5][e] = v; s_bkt[6][e] = w;
             done = true;
             bucketed = true;
@@ -595,11 +861,18 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           // list full: this thread deposits all eight nodes itself
           bool lds = false;
 #pragma unroll 1
-          for (int nd = 0; nd < 8; ++nd)
-            lds = deposit_node<T, A, BX, BY, BZ>(a.g, sp.m, s_mom, s_xn, s_yn, s_zn,
-                                                 blk.c0x, blk.c0y, blk.c0z, ix, iy, iz,
-                                                 nd >> 2, (nd >> 1) & 1, nd & 1, x, y, z,
-                                                 u, v, w, q);
+          for (int nd = 0; nd < 8; ++nd) {
+            if constexpr (FX)
+              lds = deposit_node_fx<A, BX, BY, BZ>(
+                  a.g, a.fx, sp.m, s_mom, blk.c0x, blk.c0y, blk.c0z, ix, iy, iz, nd >> 2,
+                  (nd >> 1) & 1, nd & 1, __float_as_int(x), __float_as_int(y),
+                  __float_as_int(z), u, v, w, q);
+            else
+              lds = deposit_node<T, A, BX, BY, BZ>(a.g, sp.m, s_mom, s_xn, s_yn, s_zn,
+                                                   blk.c0x, blk.c0y, blk.c0z, ix, iy, iz,
+                                                   nd >> 2, (nd >> 1) & 1, nd & 1, x, y, z,
+                                                   u, v, w, q);
+          }
           if (lds) ++n_ld;
           else ++n_gd;
           const bool stay = dest == tile;
@@ -639,8 +912,10 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           const int e = my_cell * CS + (int)k;
           const T px = s_bkt[0][e], py = s_bkt[1][e], pz = s_bkt[2][e];
           const T pq = s_bkt[3][e], pu = s_bkt[4][e], pv = s_bkt[5][e], pw = s_bkt[6][e];
-          const T xa = my_a == 0 ? px - X0 : X1 - px;
-          const T yb = my_b == 0 ? py - Y0 : Y1 - py;
+          // px py pz: the particle's position, or with FX its offsets
+          // within the cell (the distances from nodes X0 Y0 Z0 in cells)
+          const T xa = FX ? (my_a == 0 ? px : T(1) - px) : (my_a == 0 ? px - X0 : X1 - px);
+          const T yb = FX ? (my_b == 0 ? py : T(1) - py) : (my_b == 0 ? py - Y0 : Y1 - py);
           const T uu = pu * pu, uv = pu * pv, uw = pu * pw, vv = pv * pv, vw = pv * pw,
                   ww = pw * pw;
           if (kStrictB) {
@@ -650,10 +925,11 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
             add_terms<T, T>(acc, w0, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
             add_terms<T, T>(acc + 10, w1, iv, pu, pv, pw, uu, uv, uw, vv, vw, ww);
           } else {
-            // sum of p_m * (q xi eta zeta); the factor iv^2 is applied once
-            // when the sums are added to the LDS block
+            // sum of p_m * (q xi eta zeta); the factor iv^2 (with FX, whose
+            // offsets are in cells, iv) is applied once when the sums are
+            // added to the LDS block
             const T qxy = pq * xa * yb;
-            const T w0 = qxy * (pz - Z0), w1 = qxy * (Z1 - pz);
+            const T w0 = qxy * (FX ? pz : pz - Z0), w1 = qxy * (FX ? T(1) - pz : Z1 - pz);
             const T pm[10] = {T(1), pu, pv, pw, uu, uv, uw, vv, vw, ww};
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
@@ -679,9 +955,16 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
             const T pq = s_lst[3][e], pu = s_lst[4][e], pv = s_lst[5][e], pw = s_lst[6][e];
             const unsigned int c = s_lcell[e];
             const int jx = (int)(c >> 20), jy = (int)((c >> 10) & 1023u), jz = (int)(c & 1023u);
-            const bool lds = deposit_node<T, A, BX, BY, BZ>(
-                a.g, sp.m, s_mom, s_xn, s_yn, s_zn, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz,
-                nd >> 2, (nd >> 1) & 1, nd & 1, px, py, pz, pu, pv, pw, pq);
+            bool lds;
+            if constexpr (FX)
+              lds = deposit_node_fx<A, BX, BY, BZ>(
+                  a.g, a.fx, sp.m, s_mom, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz, nd >> 2,
+                  (nd >> 1) & 1, nd & 1, __float_as_int(px), __float_as_int(py),
+                  __float_as_int(pz), pu, pv, pw, pq);
+            else
+              lds = deposit_node<T, A, BX, BY, BZ>(
+                  a.g, sp.m, s_mom, s_xn, s_yn, s_zn, blk.c0x, blk.c0y, blk.c0z, jx, jy, jz,
+                  nd >> 2, (nd >> 1) & 1, nd & 1, px, py, pz, pu, pv, pw, pq);
             if (nd == 0) {
               if (lds) ++n_ld;
               else ++n_gd;
@@ -701,7 +984,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
 
     // 3. register sums into the LDS block: corner (a, b, c) of cell
     // (lx, ly, lz) is block node (lx + 2 - a, ly + 2 - b, lz + 2 - c)
-    const A scale = kStrictB ? A(1) : (A)a.g.invVOL * (A)a.g.invVOL;
+    const A scale = kStrictB ? A(1) : FX ? (A)a.g.invVOL : (A)a.g.invVOL * (A)a.g.invVOL;
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
       const int nb = ((mlx + 2 - my_a) * BY + (mly + 2 - my_b)) * BZ + (mlz + 2 - cc);
@@ -753,28 +1036,28 @@ __device__ __forceinline__ int batch_instance(const BatchMap &map, int b) {
 
 // PUSH false: the deposit alone (spic_engine_deposit), in its own
 // instances so that the phase kernels carry no test for it.
-template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true>
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __global__ __launch_bounds__(4 * TX * TY * TZ)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
 void k_binned_phase(BinnedArgs<T> a) {
-  binned_phase_tile<T, A, TX, TY, TZ, PUSH>(a, blockIdx.x);
+  binned_phase_tile<T, A, TX, TY, TZ, PUSH, FX>(a, blockIdx.x);
 }
 
 // Every instance of a batch in one grid: instance i owns blocks
 // [map.start[i], map.start[i + 1]), one per tile, with arguments tab[i].
-template <typename T, typename A, int TX, int TY, int TZ>
+template <typename T, typename A, int TX, int TY, int TZ, bool FX = false>
 __global__ __launch_bounds__(4 * TX * TY * TZ)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
 void k_binned_phase_batch(const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
   const int i = batch_instance(map, blockIdx.x);
-  binned_phase_tile<T, A, TX, TY, TZ>(tab[i], blockIdx.x - map.start[i]);
+  binned_phase_tile<T, A, TX, TY, TZ, true, FX>(tab[i], blockIdx.x - map.start[i]);
 }
 
 // The particles of every species' input overflow region: global gathers
 // and deposits, re-binned like the tile kernel's.
 // Part `part` of `nparts` (workgroups of 256 threads) takes every
 // (nparts * 256)-th particle.
-template <typename T, int TX, int TY, int TZ, bool PUSH = true>
+template <typename T, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int part,
                                                      int nparts) {
   Block<T, 1, 1, 1> none;
@@ -796,21 +1079,39 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
       T u = p[3 * st], v = p[4 * st], w = p[5 * st];
       const T q = p[6 * st];
       const uint32_t pid = gmem(sp.id_in)[i];
-      if constexpr (PUSH)
-        push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
-                                         bad_move, n_lg, n_gg);
-      int ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
-      int iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
-      int iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
-      bad_dep |= bclamp(ix, a.g.nxn);
-      bad_dep |= bclamp(iy, a.g.nyn);
-      bad_dep |= bclamp(iz, a.g.nzn);
+      int ix, iy, iz;
+      if constexpr (FX) {
+        int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
+        if constexpr (PUSH)
+          push_fixed<false, 1, 1, 1>(a.g, a.f, sp.k, a.fx, none, X, Y, Z, u, v, w, bad_move,
+                                     n_lg, n_gg);
+        ix = fx_deposit_cell(X, a.fx.k[0], a.g.nxn, bad_dep);
+        iy = fx_deposit_cell(Y, a.fx.k[1], a.g.nyn, bad_dep);
+        iz = fx_deposit_cell(Z, a.fx.k[2], a.g.nzn, bad_dep);
+        x = __int_as_float(X);
+        y = __int_as_float(Y);
+        z = __int_as_float(Z);
+      } else {
+        if constexpr (PUSH)
+          push_particle<T, false, 1, 1, 1>(a.g, a.f, sp.k, none, x, y, z, u, v, w,
+                                           bad_move, n_lg, n_gg);
+        ix = bcell_floor<T>(x, a.g.xStart, a.g.invdx, a.g.nxn);
+        iy = bcell_floor<T>(y, a.g.yStart, a.g.invdy, a.g.nyn);
+        iz = bcell_floor<T>(z, a.g.zStart, a.g.invdz, a.g.nzn);
+        bad_dep |= bclamp(ix, a.g.nxn);
+        bad_dep |= bclamp(iy, a.g.nyn);
+        bad_dep |= bclamp(iz, a.g.nzn);
+      }
       const int dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
       const int64_t o = out_slot(a, sp, dest, gatomic_add(sp.cnt_out + dest, 1u));
       if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
       else bad_store = true;
       ++n_gd;
-      deposit_global<T>(a.g, sp.m, ix, iy, iz, x, y, z, u, v, w, q);
+      if constexpr (FX)
+        deposit_global_fx(a.g, a.fx, sp.m, ix, iy, iz, __float_as_int(x), __float_as_int(y),
+                          __float_as_int(z), u, v, w, q);
+      else
+        deposit_global<T>(a.g, sp.m, ix, iy, iz, x, y, z, u, v, w, q);
     }
   }
   if (bad_move) gatomic_add(a.status, 1u);
@@ -822,30 +1123,30 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
   }
 }
 
-template <typename T, int TX, int TY, int TZ, bool PUSH = true>
+template <typename T, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
-  binned_overflow_part<T, TX, TY, TZ, PUSH>(a, blockIdx.x, gridDim.x);
+  binned_overflow_part<T, TX, TY, TZ, PUSH, FX>(a, blockIdx.x, gridDim.x);
 }
 
-template <typename T, int TX, int TY, int TZ>
+template <typename T, int TX, int TY, int TZ, bool FX = false>
 __global__ __launch_bounds__(256) void k_binned_overflow_batch(
     const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
   const int i = batch_instance(map, blockIdx.x);
-  binned_overflow_part<T, TX, TY, TZ>(tab[i], blockIdx.x - map.start[i],
-                                      map.start[i + 1] - map.start[i]);
+  binned_overflow_part<T, TX, TY, TZ, true, FX>(tab[i], blockIdx.x - map.start[i],
+                                                map.start[i + 1] - map.start[i]);
 }
 
-template <typename T, typename A, bool PUSH>
+template <typename T, typename A, bool PUSH, bool FX>
 hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
   if (a.tx == 4 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH>), dim3(a.ntiles), dim3(256), 0,
-                       s, a);
-    hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4, PUSH>), dim3(a.n_ovf_blocks),
+    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(256),
+                       0, s, a);
+    hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4, PUSH, FX>), dim3(a.n_ovf_blocks),
                        dim3(256), 0, s, a);
   } else if (a.tx == 8 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH>), dim3(a.ntiles), dim3(512), 0,
-                       s, a);
-    hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4, PUSH>), dim3(a.n_ovf_blocks),
+    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(512),
+                       0, s, a);
+    hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4, PUSH, FX>), dim3(a.n_ovf_blocks),
                        dim3(256), 0, s, a);
   } else {
     return hipErrorInvalidValue;
@@ -853,51 +1154,74 @@ hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename A, bool PUSH>
+hipError_t launch_binned_pos(const BinnedArgs<T> &a, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && !kStrictB) {
+    if (a.fixed) return launch_binned_shape<T, A, PUSH, true>(a, s);
+  } else {
+    // fixed-point positions exist only in FP32 production arithmetic; the
+    // engine refuses the other combinations before launching
+    if (a.fixed) return hipErrorInvalidValue;
+  }
+  return launch_binned_shape<T, A, PUSH, false>(a, s);
+}
+
 template <typename T>
 hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
   if (a.ntiles <= 0 || a.ns <= 0) return hipSuccess;
   const bool lds64 = sizeof(T) == 4 && a.lds_fp64;
   if (a.push_off)
-    return lds64 ? launch_binned_shape<T, double, false>(a, s)
-                 : launch_binned_shape<T, T, false>(a, s);
-  return lds64 ? launch_binned_shape<T, double, true>(a, s)
-               : launch_binned_shape<T, T, true>(a, s);
+    return lds64 ? launch_binned_pos<T, double, false>(a, s)
+                 : launch_binned_pos<T, T, false>(a, s);
+  return lds64 ? launch_binned_pos<T, double, true>(a, s)
+               : launch_binned_pos<T, T, true>(a, s);
 }
 
 template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);
 template hipError_t launch_binned_phase<float>(const BinnedArgs<float> &, hipStream_t);
 
-template <typename T, typename A, int TX, int TY, int TZ>
+template <typename T, typename A, int TX, int TY, int TZ, bool FX>
 hipError_t launch_batch_shape(const BinnedArgs<T> *tab, const BatchMap &tiles,
                               const BatchMap &ovf, hipStream_t s) {
-  hipLaunchKernelGGL((k_binned_phase_batch<T, A, TX, TY, TZ>), dim3(tiles.start[tiles.n]),
-                     dim3(4 * TX * TY * TZ), 0, s, tab, tiles);
-  hipLaunchKernelGGL((k_binned_overflow_batch<T, TX, TY, TZ>), dim3(ovf.start[ovf.n]),
+  hipLaunchKernelGGL((k_binned_phase_batch<T, A, TX, TY, TZ, FX>),
+                     dim3(tiles.start[tiles.n]), dim3(4 * TX * TY * TZ), 0, s, tab, tiles);
+  hipLaunchKernelGGL((k_binned_overflow_batch<T, TX, TY, TZ, FX>), dim3(ovf.start[ovf.n]),
                      dim3(256), 0, s, tab, ovf);
   return hipGetLastError();
+}
+
+template <typename T, typename A, int TX, int TY, int TZ>
+hipError_t launch_batch_pos(const BinnedArgs<T> *tab, const BatchMap &tiles,
+                            const BatchMap &ovf, int fixed, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && !kStrictB) {
+    if (fixed) return launch_batch_shape<T, A, TX, TY, TZ, true>(tab, tiles, ovf, s);
+  } else {
+    if (fixed) return hipErrorInvalidValue;
+  }
+  return launch_batch_shape<T, A, TX, TY, TZ, false>(tab, tiles, ovf, s);
 }
 
 template <typename T>
 hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab, const BatchMap &tiles,
                                      const BatchMap &ovf, int tx, int ty, int tz,
-                                     int lds_fp64, hipStream_t s) {
+                                     int lds_fp64, int fixed, hipStream_t s) {
   if (tiles.n <= 0 || tiles.start[tiles.n] <= 0) return hipSuccess;
   const bool wide = sizeof(T) == 4 && lds_fp64;
   if (tx == 4 && ty == 4 && tz == 4)
-    return wide ? launch_batch_shape<T, double, 4, 4, 4>(tab, tiles, ovf, s)
-                : launch_batch_shape<T, T, 4, 4, 4>(tab, tiles, ovf, s);
+    return wide ? launch_batch_pos<T, double, 4, 4, 4>(tab, tiles, ovf, fixed, s)
+                : launch_batch_pos<T, T, 4, 4, 4>(tab, tiles, ovf, fixed, s);
   if (tx == 8 && ty == 4 && tz == 4)
-    return wide ? launch_batch_shape<T, double, 8, 4, 4>(tab, tiles, ovf, s)
-                : launch_batch_shape<T, T, 8, 4, 4>(tab, tiles, ovf, s);
+    return wide ? launch_batch_pos<T, double, 8, 4, 4>(tab, tiles, ovf, fixed, s)
+                : launch_batch_pos<T, T, 8, 4, 4>(tab, tiles, ovf, fixed, s);
   return hipErrorInvalidValue;
 }
 
 template hipError_t launch_binned_phase_batch<double>(const BinnedArgs<double> *,
                                                       const BatchMap &, const BatchMap &,
-                                                      int, int, int, int, hipStream_t);
+                                                      int, int, int, int, int, hipStream_t);
 template hipError_t launch_binned_phase_batch<float>(const BinnedArgs<float> *,
                                                      const BatchMap &, const BatchMap &,
-                                                     int, int, int, int, hipStream_t);
+                                                     int, int, int, int, int, hipStream_t);
 
 }  // namespace SPIC_NS
 }  // namespace spic

@@ -35,11 +35,26 @@ inline unsigned nblocks(long n, int tpb = 256) {
 
 inline size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
 
-template <typename T>
-__device__ __forceinline__ int phys_cell(T p, double start, T inv, int nc) {
+template <typename TS, typename T>
+__device__ __forceinline__ int phys_cell(TS p, double start, T inv, int nc) {
   double c = floor(((double)p - start) * (double)inv);
   c = fmin(fmax(c, 0.0), (double)(nc - 1));
   return (int)c;
+}
+
+// Fixed-point positions (FixedParams): a length from the caller's arrays
+// to the int32 coordinate along axis a, rounded to nearest and bounded to
+// the box plus 32 cells either side (a position farther out is reported
+// by the phase as out of the grid); and back.
+__device__ __forceinline__ int to_fixed_pos(double p, const FixedParams &f, int a) {
+  const double ext = (double)(32 << f.k[a]);
+  double X = rint((p - f.start[a]) * f.to_fixed[a]);
+  X = fmin(fmax(X, -ext), (double)f.lim[a] + ext);
+  return (int)X;
+}
+
+__device__ __forceinline__ double from_fixed_pos(int X, const FixedParams &f, int a) {
+  return f.start[a] + (double)X * f.from_fixed[a];
 }
 
 // Particles per range of a binned source (tiles, then the overflow region).
@@ -76,25 +91,38 @@ __device__ __forceinline__ uint32_t wave_rank(unsigned int *counter, uint32_t ke
 }
 
 // Key of one particle: tile | rank of the particle within its cell | cell
-// within the tile.
-template <typename T>
-__device__ __forceinline__ void cell_of(const KGrid<T> &g, const BinLayout &L, T x, T y, T z,
-                                        int &cx, int &cy, int &cz, uint32_t &tile,
-                                        uint32_t &cit) {
-  cx = phys_cell<T>(x, g.xStart, g.invdx, L.ncx);
-  cy = phys_cell<T>(y, g.yStart, g.invdy, L.ncy);
-  cz = phys_cell<T>(z, g.zStart, g.invdz, L.ncz);
+// within the tile.  `binned`: the position comes from engine storage, in
+// fixed point when the layout says so (else from the caller's arrays).
+template <typename TS, typename T>
+__device__ __forceinline__ void cell_of(const KGrid<T> &g, const BinLayout &L, TS x, TS y,
+                                        TS z, int &cx, int &cy, int &cz, uint32_t &tile,
+                                        uint32_t &cit, bool binned = true) {
+  if constexpr (sizeof(TS) == 4) {
+    if (binned && L.fixed) {
+      cx = min(max(__float_as_int((float)x) >> L.fx.k[0], 0), L.ncx - 1);
+      cy = min(max(__float_as_int((float)y) >> L.fx.k[1], 0), L.ncy - 1);
+      cz = min(max(__float_as_int((float)z) >> L.fx.k[2], 0), L.ncz - 1);
+    } else {
+      cx = phys_cell(x, g.xStart, g.invdx, L.ncx);
+      cy = phys_cell(y, g.yStart, g.invdy, L.ncy);
+      cz = phys_cell(z, g.zStart, g.invdz, L.ncz);
+    }
+  } else {
+    cx = phys_cell(x, g.xStart, g.invdx, L.ncx);
+    cy = phys_cell(y, g.yStart, g.invdy, L.ncy);
+    cz = phys_cell(z, g.zStart, g.invdz, L.ncz);
+  }
   const int tx = cx / L.tx, ty = cy / L.ty, tz = cz / L.tz;
   tile = (uint32_t)((tx * L.nty + ty) * L.ntz + tz);
   cit = (uint32_t)(((cx - tx * L.tx) * L.ty + (cy - ty * L.ty)) * L.tz + (cz - tz * L.tz));
 }
 
-// Keys of a dense source (particles 0..n-1 in the caller's order), ranks
-// from per-cell counters with wave-aggregated atomics.
-template <typename T>
+// Keys of a dense source (particles 0..n-1 in the caller's order, element
+// type TS), ranks from per-cell counters with wave-aggregated atomics.
+template <typename TS, typename T>
 __global__ void k_bin_keys(KGrid<T> g, BinLayout L, int64_t n,
-                           const T *__restrict__ x, const T *__restrict__ y,
-                           const T *__restrict__ z, unsigned int *__restrict__ cell_count,
+                           const TS *__restrict__ x, const TS *__restrict__ y,
+                           const TS *__restrict__ z, unsigned int *__restrict__ cell_count,
                            uint32_t *__restrict__ key, uint32_t *__restrict__ idx) {
   const uint32_t rank_max = (1u << L.rank_bits) - 1u;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -103,7 +131,7 @@ __global__ void k_bin_keys(KGrid<T> g, BinLayout L, int64_t n,
     const bool active = i < n;
     int cx = 0, cy = 0, cz = 0;
     uint32_t tile = 0, cit = 0;
-    if (active) cell_of<T>(g, L, x[i], y[i], z[i], cx, cy, cz, tile, cit);
+    if (active) cell_of(g, L, x[i], y[i], z[i], cx, cy, cz, tile, cit, false);
     const uint32_t cell = (uint32_t)(((int64_t)cx * L.ncy + cy) * L.ncz + cz);
     uint32_t r = wave_rank(cell_count, cell, active);
     if (!active) continue;
@@ -137,7 +165,7 @@ __global__ __launch_bounds__(256) void k_bin_keys_binned(
     const int64_t j = v.slot(r);
     int cx, cy, cz;
     uint32_t tile, cit;
-    cell_of<T>(g, L, x[j], y[j], z[j], cx, cy, cz, tile, cit);
+    cell_of(g, L, x[j], y[j], z[j], cx, cy, cz, tile, cit);
     uint32_t rk;
     if ((int)tile == t) rk = atomicAdd(&s_cnt[cit], 1u);
     else rk = atomicAdd(&cell_count[((size_t)cx * L.ncy + cy) * L.ncz + cz], 1u);
@@ -162,18 +190,30 @@ struct Words6 {
   uint32_t *p[6];
 };
 
-template <typename T>
-__global__ void k_bin_gather(Arrays7<T> src, const uint32_t *__restrict__ src_id,
+// The sorted gather into binned storage.  A dense source (src_id null)
+// holds the caller's values in TS: with fixed-point storage (to_fixed) its
+// positions are converted, the rest rounded to T; a binned source is the
+// engine's own storage, copied word for word.
+template <typename TS, typename T>
+__global__ void k_bin_gather(Arrays7<TS> src, const uint32_t *__restrict__ src_id,
                              const uint32_t *__restrict__ perm,
                              const uint32_t *__restrict__ keys, int shift, int64_t n,
-                             int64_t slack, Arrays7<T> dst,
-                             uint32_t *__restrict__ dst_id) {
+                             int64_t slack, Arrays7<T> dst, uint32_t *__restrict__ dst_id,
+                             int to_fixed, FixedParams fx) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t j = perm[i];
     const int64_t o = i + (int64_t)(keys[i] >> shift) * slack;
 #pragma unroll
-    for (int a = 0; a < 7; ++a) dst.p[a][o] = src.p[a][j];
+    for (int a = 0; a < 7; ++a) {
+      if constexpr (sizeof(T) == 4) {
+        if (a < 3 && to_fixed) {
+          dst.p[a][o] = __int_as_float(to_fixed_pos((double)src.p[a][j], fx, a));
+          continue;
+        }
+      }
+      dst.p[a][o] = (T)src.p[a][j];
+    }
     dst_id[o] = src_id ? src_id[j] : j;
   }
 }
@@ -203,18 +243,28 @@ __global__ void k_bin_bounds(const uint32_t *__restrict__ keys, int shift, int64
   }
 }
 
-// One workgroup per segment (then the overflow region): dst[id] = src.
-template <typename T>
+// One workgroup per segment (then the overflow region): dst[id] = src, in
+// the caller's element type TD; fixed-point positions converted back.
+template <typename T, typename TD>
 __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
                             const int64_t *__restrict__ seg,
                             const unsigned int *__restrict__ cnt,
-                            unsigned int ovf_cap, int ntiles, Arrays7<T> dst) {
+                            unsigned int ovf_cap, int ntiles, Arrays7<TD> dst, int fixed,
+                            FixedParams fx) {
   const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
   for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
     const int64_t i = v.slot(r);
     const uint32_t k = id[i];
 #pragma unroll
-    for (int a = 0; a < 7; ++a) dst.p[a][k] = src.p[a][i];
+    for (int a = 0; a < 7; ++a) {
+      if constexpr (sizeof(T) == 4) {
+        if (a < 3 && fixed) {
+          dst.p[a][k] = (TD)from_fixed_pos(__float_as_int((float)src.p[a][i]), fx, a);
+          continue;
+        }
+      }
+      dst.p[a][k] = (TD)src.p[a][i];
+    }
   }
 }
 
@@ -230,13 +280,21 @@ template <typename T>
 __global__ void k_bin_store_be32(Arrays7<T> src, const uint32_t *__restrict__ id,
                                  const int64_t *__restrict__ seg,
                                  const unsigned int *__restrict__ cnt, unsigned int ovf_cap,
-                                 int ntiles, Words6 dst) {
+                                 int ntiles, Words6 dst, int fixed, FixedParams fx) {
   const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
   for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
     const int64_t i = v.slot(r);
     const uint32_t k = id[i];
 #pragma unroll
-    for (int a = 0; a < 6; ++a) dst.p[a][k] = be32(src.p[a][i]);
+    for (int a = 0; a < 6; ++a) {
+      if constexpr (sizeof(T) == 4) {
+        if (a < 3 && fixed) {
+          dst.p[a][k] = be32(from_fixed_pos(__float_as_int((float)src.p[a][i]), fx, a));
+          continue;
+        }
+      }
+      dst.p[a][k] = be32(src.p[a][i]);
+    }
   }
 }
 
@@ -496,7 +554,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_count(
       if (in) {
         int cx, cy, cz;
         uint32_t cit;
-        cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
+        cell_of(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
       }
       const bool home = in && !ovf && (int)tile == b;
       mine += home ? 1u : 0u;
@@ -634,7 +692,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
         j = v.slot(r);
         int cx, cy, cz;
         uint32_t cit;
-        cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
+        cell_of(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
                    cit);
       }
       const uint32_t k = wave_rank_g(fill, tile, in);
@@ -672,7 +730,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
       if (r < v.count) {
         int cx, cy, cz;
         uint32_t tile, cit;
-        cell_of<T>(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
+        cell_of(g, L, px[u], py[u], pz[u], cx, cy, cz, tile, cit);
         const bool home = (int)tile == t;
         if (home) atomicAdd(&s_count[cit], 1u);
         if (staged) s_slot[r] = home ? (uint16_t)cit : kForeign;
@@ -714,7 +772,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
         const int64_t j = v.slot(r);
         int cx, cy, cz;
         uint32_t cit;
-        cell_of<T>(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
+        cell_of(g, L, src[j], src[stride + j], src[2 * stride + j], cx, cy, cz, tile,
                    cit);
         c = (int)tile == t ? cit : kForeign;
       }
@@ -884,13 +942,17 @@ hipError_t zero_counts(const CountArrays &c, hipStream_t s) {
 
 template <typename T>
 hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
-                    T *const src[7], const uint32_t *src_id,
+                    const void *const src[7], int src_f64, const uint32_t *src_id,
                     const int64_t *src_seg, const unsigned int *src_cnt,
                     unsigned int src_ovf_cap, T *const dst[7], uint32_t *dst_id,
                     int64_t *dst_seg, unsigned int *dst_cnt, void *workspace,
                     size_t workspace_bytes, size_t *needed, hipStream_t s) {
   if (n < 0 || n > 0xFFFFFFFFll) return hipErrorInvalidValue;
   const bool binned = src_seg != nullptr;
+  // the caller's FP64 arrays into an FP32 engine (a binned source is always
+  // the engine's own precision)
+  const bool widened = !binned && src_f64 && sizeof(T) == 4;
+  if (binned && src_f64 != (sizeof(T) == 8 ? 1 : 0)) return hipErrorInvalidValue;
   const size_t nt1 = (size_t)L.ntiles + 1;
   const size_t a_n = align_up(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
   const size_t a_t = align_up(sizeof(uint32_t) * nt1);
@@ -940,23 +1002,35 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
                          (const T *)src[0], (const T *)src[1], (const T *)src[2],
                          (const unsigned int *)P, src_seg, src_cnt, src_ovf_cap, cell_count,
                          keys_in, idx_in);
+    else if (widened)
+      hipLaunchKernelGGL((k_bin_keys<double, T>), dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
+                         (const double *)src[0], (const double *)src[1],
+                         (const double *)src[2], cell_count, keys_in, idx_in);
     else
-      hipLaunchKernelGGL(k_bin_keys<T>, dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
+      hipLaunchKernelGGL((k_bin_keys<T, T>), dim3(nblocks(n)), dim3(256), 0, s, g, L, n,
                          (const T *)src[0], (const T *)src[1], (const T *)src[2],
                          cell_count, keys_in, idx_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys_in, keys_out, idx_in,
                                   perm, (size_t)n, 0, key_bits, s);
     if (e != hipSuccess) return e;
-    Arrays7<T> as, ad;
-    for (int a = 0; a < 7; ++a) {
-      as.p[a] = src[a];
-      ad.p[a] = dst[a];
+    Arrays7<T> ad;
+    for (int a = 0; a < 7; ++a) ad.p[a] = dst[a];
+    const int conv = (!binned && L.fixed) ? 1 : 0;
+    const uint32_t *sid = binned ? src_id : (const uint32_t *)nullptr;
+    if (widened) {
+      Arrays7<double> as;
+      for (int a = 0; a < 7; ++a) as.p[a] = (double *)src[a];
+      hipLaunchKernelGGL((k_bin_gather<double, T>), dim3(nblocks(n)), dim3(256), 0, s, as,
+                         sid, (const uint32_t *)perm, (const uint32_t *)keys_out, shift, n,
+                         L.slack, ad, dst_id, conv, L.fx);
+    } else {
+      Arrays7<T> as;
+      for (int a = 0; a < 7; ++a) as.p[a] = (T *)src[a];
+      hipLaunchKernelGGL((k_bin_gather<T, T>), dim3(nblocks(n)), dim3(256), 0, s, as, sid,
+                         (const uint32_t *)perm, (const uint32_t *)keys_out, shift, n,
+                         L.slack, ad, dst_id, conv, L.fx);
     }
-    hipLaunchKernelGGL(k_bin_gather<T>, dim3(nblocks(n)), dim3(256), 0, s, as,
-                       binned ? src_id : (const uint32_t *)nullptr,
-                       (const uint32_t *)perm, (const uint32_t *)keys_out, shift, n,
-                       L.slack, ad, dst_id);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_bin_bounds, dim3(nblocks((long)nt1)), dim3(256), 0, s,
@@ -966,13 +1040,13 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
 }
 
 template hipError_t bin_sort<double>(const KGrid<double> &, const BinLayout &, int64_t,
-                                     double *const[7], const uint32_t *,
+                                     const void *const[7], int, const uint32_t *,
                                      const int64_t *, const unsigned int *,
                                      unsigned int, double *const[7], uint32_t *,
                                      int64_t *, unsigned int *, void *, size_t,
                                      size_t *, hipStream_t);
 template hipError_t bin_sort<float>(const KGrid<float> &, const BinLayout &, int64_t,
-                                    float *const[7], const uint32_t *,
+                                    const void *const[7], int, const uint32_t *,
                                     const int64_t *, const unsigned int *,
                                     unsigned int, float *const[7], uint32_t *,
                                     int64_t *, unsigned int *, void *, size_t,
@@ -981,14 +1055,21 @@ template hipError_t bin_sort<float>(const KGrid<float> &, const BinLayout &, int
 template <typename T>
 hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
                      const int64_t *seg, const unsigned int *cnt,
-                     unsigned int ovf_cap, T *const dst[7], hipStream_t s) {
-  Arrays7<T> as, ad;
-  for (int a = 0; a < 7; ++a) {
-    as.p[a] = src[a];
-    ad.p[a] = dst[a];
+                     unsigned int ovf_cap, void *const dst[7], int dst_f64, hipStream_t s) {
+  Arrays7<T> as;
+  for (int a = 0; a < 7; ++a) as.p[a] = src[a];
+  if (dst_f64) {
+    Arrays7<double> ad;
+    for (int a = 0; a < 7; ++a) ad.p[a] = (double *)dst[a];
+    hipLaunchKernelGGL((k_bin_store<T, double>), dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
+                       seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
+  } else {
+    if (sizeof(T) == 8) return hipErrorInvalidValue;  // no narrowing store
+    Arrays7<float> ad;
+    for (int a = 0; a < 7; ++a) ad.p[a] = (float *)dst[a];
+    hipLaunchKernelGGL((k_bin_store<T, float>), dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
+                       seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
   }
-  hipLaunchKernelGGL(k_bin_store<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
-                     seg, cnt, ovf_cap, L.ntiles, ad);
   return hipGetLastError();
 }
 
@@ -1001,7 +1082,7 @@ hipError_t bin_store_be32(const BinLayout &L, T *const src[7], const uint32_t *i
   for (int a = 0; a < 7; ++a) as.p[a] = src[a];
   for (int a = 0; a < 6; ++a) ad.p[a] = dst[a];
   hipLaunchKernelGGL(k_bin_store_be32<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id, seg,
-                     cnt, ovf_cap, L.ntiles, ad);
+                     cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
   return hipGetLastError();
 }
 
@@ -1017,10 +1098,10 @@ template hipError_t bin_store_be32<float>(const BinLayout &, float *const[7],
 template hipError_t bin_store<double>(const BinLayout &, double *const[7],
                                       const uint32_t *, const int64_t *,
                                       const unsigned int *, unsigned int,
-                                      double *const[7], hipStream_t);
+                                      void *const[7], int, hipStream_t);
 template hipError_t bin_store<float>(const BinLayout &, float *const[7],
                                      const uint32_t *, const int64_t *,
                                      const unsigned int *, unsigned int,
-                                     float *const[7], hipStream_t);
+                                     void *const[7], int, hipStream_t);
 
 }  // namespace spic

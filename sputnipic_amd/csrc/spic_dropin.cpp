@@ -242,6 +242,9 @@ spic_engine *grid_engine(GridMirror *gm) {
   if (!gm->eng && !gm->eng_failed) {
     spic_engine_opts o;
     std::memset(&o, 0, sizeof(o));
+    // the caller's arrays hold the particles between calls in the
+    // reference's own representation, so the engine keeps it too
+    o.positions = SPIC_POS_FLOAT;
     if (spic_engine_create(&gm->eng, &gm->g, 1, &o) != SPIC_OK) {
       gm->eng = nullptr;
       gm->eng_failed = true;

@@ -3,6 +3,7 @@
 // Host code only; the kernels are in spic_binned.inl and spic_binsort.hip.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -65,7 +66,16 @@ int ceil_log2(long v) {
   return b;
 }
 
+// spic_test_fail_engine_alloc: allocations left before the one that fails
+// (negative: none fails)
+std::atomic<int> g_fail_after{-1};
+
 hipError_t dmalloc(spic_engine *e, void **p, size_t bytes) {
+  *p = nullptr;
+  if (g_fail_after.load() >= 0 && g_fail_after.fetch_sub(1) == 0) {
+    g_fail_after.store(-1);
+    return hipErrorOutOfMemory;
+  }
   hipError_t err = hipMalloc(p, bytes ? bytes : 1);
   if (err == hipSuccess) e->bytes += bytes;
   return err;
@@ -155,8 +165,10 @@ hipError_t tile_sort_species(spic_engine *e, int is, hipStream_t s) {
   return hipSuccess;
 }
 
+// `src`: the caller's arrays (src_f64: FP64 ones) for a load, else null.
 template <typename T>
-hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
+hipError_t sort_species(spic_engine *e, int is, void *const *src, int src_f64,
+                        hipStream_t s) {
   auto &sp = e->sp[is];
   if (!src && e->sort_mode == SPIC_SORT_TILE) return tile_sort_species<T>(e, is, s);
   const spic::KGrid<T> g = spic::kgrid<T>(&e->grd);
@@ -164,21 +176,23 @@ hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
   L.slack = sp.slack;
   const int from = sp.cur, to = src ? 0 : 1 - sp.cur;
   const int lay_from = sp.segi, lay_to = src ? 0 : 1 - sp.segi;
-  T *in[7], *out[7];
+  const void *in[7];
+  T *out[7];
   for (int a = 0; a < 7; ++a) {
-    in[a] = src ? src[a] : (T *)sp.buf[from][a];
+    in[a] = src ? src[a] : sp.buf[from][a];
     out[a] = (T *)sp.buf[to][a];
   }
+  const int in_f64 = src ? src_f64 : (sizeof(T) == 8 ? 1 : 0);
   const uint32_t *in_id = src ? nullptr : sp.id[from];
   const int64_t *in_seg = src ? nullptr : sp.seg[lay_from];
   const unsigned int *in_cnt = src ? nullptr : sp.cnt[from];
   size_t need = 0;
-  hipError_t err = spic::bin_sort<T>(g, L, sp.n, in, in_id, in_seg, in_cnt, sp.ovf_cap,
-                                     out, sp.id[to], sp.seg[lay_to], sp.cnt[to],
+  hipError_t err = spic::bin_sort<T>(g, L, sp.n, in, in_f64, in_id, in_seg, in_cnt,
+                                     sp.ovf_cap, out, sp.id[to], sp.seg[lay_to], sp.cnt[to],
                                      nullptr, 0, &need, s);
   if (err != hipSuccess) return err;
   if ((err = ensure_workspace(e, need, s)) != hipSuccess) return err;
-  err = spic::bin_sort<T>(g, L, sp.n, in, in_id, in_seg, in_cnt, sp.ovf_cap, out,
+  err = spic::bin_sort<T>(g, L, sp.n, in, in_f64, in_id, in_seg, in_cnt, sp.ovf_cap, out,
                           sp.id[to], sp.seg[lay_to], sp.cnt[to], e->ws, e->ws_bytes,
                           nullptr, s);
   if (err != hipSuccess) return err;
@@ -188,10 +202,10 @@ hipError_t sort_species(spic_engine *e, int is, T *const *src, hipStream_t s) {
   return hipSuccess;
 }
 
-hipError_t sort_any(spic_engine *e, int is, void *const *src, hipStream_t s) {
-  return e->grd.prec == SPIC_F64
-             ? sort_species<double>(e, is, (double *const *)src, s)
-             : sort_species<float>(e, is, (float *const *)src, s);
+hipError_t sort_any(spic_engine *e, int is, void *const *src, hipStream_t s,
+                    int src_f64 = 0) {
+  return e->grd.prec == SPIC_F64 ? sort_species<double>(e, is, src, 1, s)
+                                 : sort_species<float>(e, is, src, src_f64, s);
 }
 
 // The phase kernel's argument block for the loaded species of `e`, read
@@ -209,6 +223,8 @@ int fill_args(const spic_engine *e, const spic_field *field, const spic_moments 
   a.ntiles = e->L.ntiles;
   a.tx = e->L.tx; a.ty = e->L.ty; a.tz = e->L.tz;
   a.lds_fp64 = e->lds_fp64;
+  a.fixed = e->L.fixed;
+  a.fx = e->L.fx;
   a.n_ovf_blocks = kOverflowBlocks;
   a.status = status;
   a.counters = e->counters;
@@ -252,8 +268,6 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   spic::BinnedArgs<T> a;
   int idx[spic::kMaxSpecies];
   const int k = fill_args<T>(e, field, mom, push, status, -1, -1, a, idx);
-  if (k == 0) return hipSuccess;
-  a.push_off = push_off;
   spic::BinnedReduceArgs r;
   std::memset(&r, 0, sizeof(r));
   spic::CountArrays zc;
@@ -264,12 +278,25 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
     zc.mom[j] = a.sp[j].m;
   }
   r.ns = zc.ns = k;
-  zc.n = 3 * (e->L.ntiles + 1);
   // writing the moments: zero them in the same launch; the deposits (LDS
-  // partials through the reduce, global atomics directly) then add
+  // partials through the reduce, global atomics directly) then add.  A
+  // loaded species without particles deposits nothing, so its moments are
+  // zeroed here too (its counters, which no kernel reads, along with them).
+  if (e->set_moments) {
+    for (int is = 0; is < e->ns; ++is) {
+      const auto &sp = e->sp[is];
+      if (!sp.loaded || sp.n != 0) continue;
+      zc.p[zc.ns] = sp.cnt[1 - sp.cur];
+      zc.mom[zc.ns] = spic::kmom(&mom[is]);
+      ++zc.ns;
+    }
+  }
+  if (zc.ns == 0) return hipSuccess;
+  a.push_off = push_off;
+  zc.n = 3 * (e->L.ntiles + 1);
   zc.nn = e->set_moments ? (int64_t)e->grd.nxn * e->grd.nyn * e->grd.nzn : 0;
   hipError_t err = spic::zero_counts(zc, s);
-  if (err != hipSuccess) return err;
+  if (err != hipSuccess || k == 0) return err;
   err = spic::strict_mode() ? spic::spic_strict::launch_binned_phase<T>(a, s)
                             : spic::spic_fast::launch_binned_phase<T>(a, s);
   if (err != hipSuccess) return err;
@@ -286,6 +313,17 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   return hipSuccess;
 }
 
+// Strict arithmetic reproduces the reference's rounding of lengths, which
+// fixed-point storage does not hold.
+int check_positions(const spic_engine *e) {
+  if (e->L.fixed && spic::strict_mode())
+    return fail(SPIC_EINVAL,
+                "the engine stores fixed-point positions (an FP32 engine created in "
+                "production arithmetic); strict arithmetic needs one created with "
+                "positions = SPIC_POS_FLOAT");
+  return SPIC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -299,7 +337,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
   int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0,
-      sort_mode = SPIC_SORT_TILE;
+      sort_mode = SPIC_SORT_TILE, positions = 0;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -317,7 +355,17 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->sort_mode != SPIC_SORT_TILE && opts->sort_mode != SPIC_SORT_RADIX)
       return fail(SPIC_EINVAL, "sort_mode must be SPIC_SORT_TILE or SPIC_SORT_RADIX");
     sort_mode = opts->sort_mode;
+    if (opts->positions != 0 && opts->positions != SPIC_POS_FLOAT &&
+        opts->positions != SPIC_POS_FIXED)
+      return fail(SPIC_EINVAL, "positions must be 0, SPIC_POS_FLOAT or SPIC_POS_FIXED");
+    positions = opts->positions;
   }
+  if (positions == 0)
+    positions = (grd->prec == SPIC_F32 && !spic::strict_mode()) ? SPIC_POS_FIXED
+                                                                 : SPIC_POS_FLOAT;
+  if (positions == SPIC_POS_FIXED && grd->prec != SPIC_F32)
+    return fail(SPIC_EUNSUPPORTED,
+                "fixed-point positions are for FP32 engines (an FP64 length is finer)");
   // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
   if (grd->nxn > 4096 || grd->nyn > 1024 || grd->nzn > 1024)
     return fail(SPIC_EUNSUPPORTED,
@@ -352,6 +400,9 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
                 L.ntiles);
   }
   if (L.rank_bits > 16) L.rank_bits = 16;
+  L.fixed = positions == SPIC_POS_FIXED ? 1 : 0;
+  L.fx = spic::fixed_params(L.ncx, L.ncy, L.ncz, grd->dx, grd->dy, grd->dz, grd->xStart,
+                            grd->yStart, grd->zStart);
   // copy the node tables
   hipError_t err;
   const size_t sx = (size_t)e->esz * grd->nxn, sy = (size_t)e->esz * grd->nyn,
@@ -400,9 +451,13 @@ int spic_engine_destroy(spic_engine *e) {
   return SPIC_OK;
 }
 
-int spic_engine_load(spic_engine *e, int is, const spic_species *src,
-                     void *stream) {
+int spic_engine_load_as(spic_engine *e, int is, const spic_species *src, int prec,
+                        void *stream) {
   if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  if (prec != SPIC_F64 && prec != SPIC_F32)
+    return fail(SPIC_EINVAL, "precision must be SPIC_F64 or SPIC_F32");
+  if (e->grd.prec == SPIC_F64 && prec != SPIC_F64)
+    return fail(SPIC_EUNSUPPORTED, "an FP64 engine loads FP64 arrays only");
   if (is < 0 || is >= e->ns) return fail(SPIC_EINVAL, "species %d out of range", is);
   int rc = spic::check_species(src, true, true);
   if (rc) return rc;
@@ -424,16 +479,24 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
     if ((err = hipDeviceSynchronize()) != hipSuccess) return hip_fail(err, "engine load");
     free_species(e, sp);
     sp.alloc_cap = cap;
-    for (int b = 0; b < 2; ++b) {
+    // a failed allocation frees every buffer this load allocated (free_species
+    // releases the non-null ones), so the species is left unallocated and the
+    // next load allocates all of them again instead of reusing a partial set
+    const char *what = nullptr;
+    for (int b = 0; b < 2 && !what; ++b) {
       if ((err = dmalloc(e, &sp.block[b], (size_t)e->esz * 7 * cap)) != hipSuccess)
-        return hip_fail(err, "engine particle storage");
-      if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
-        return hip_fail(err, "engine id storage");
-      if ((err = dmalloc(e, (void **)&sp.cnt[b], sizeof(unsigned int) * 3 * (nt + 1))) !=
-              hipSuccess ||
-          (err = dmalloc(e, (void **)&sp.seg[b], sizeof(int64_t) * (nt + 1))) !=
-              hipSuccess)
-        return hip_fail(err, "engine tile tables");
+        what = "engine particle storage";
+      else if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
+        what = "engine id storage";
+      else if ((err = dmalloc(e, (void **)&sp.cnt[b],
+                              sizeof(unsigned int) * 3 * (nt + 1))) != hipSuccess ||
+               (err = dmalloc(e, (void **)&sp.seg[b], sizeof(int64_t) * (nt + 1))) !=
+                   hipSuccess)
+        what = "engine tile tables";
+    }
+    if (what) {
+      free_species(e, sp);
+      return hip_fail(err, what);
     }
   }
   for (int b = 0; b < 2; ++b)
@@ -451,7 +514,7 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
   sp.loaded = true;
   e->gen++;
   void *srcs[7] = {src->x, src->y, src->z, src->u, src->v, src->w, src->q};
-  if ((err = sort_any(e, is, srcs, s)) != hipSuccess) {
+  if ((err = sort_any(e, is, srcs, s, prec == SPIC_F64 ? 1 : 0)) != hipSuccess) {
     sp.loaded = false;
     return hip_fail(err, "engine sort");
   }
@@ -460,8 +523,18 @@ int spic_engine_load(spic_engine *e, int is, const spic_species *src,
   return SPIC_OK;
 }
 
-int spic_engine_store(spic_engine *e, int is, spic_species *dst, void *stream) {
+int spic_engine_load(spic_engine *e, int is, const spic_species *src, void *stream) {
   if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  return spic_engine_load_as(e, is, src, e->grd.prec, stream);
+}
+
+int spic_engine_store_as(spic_engine *e, int is, spic_species *dst, int prec,
+                         void *stream) {
+  if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  if (prec != SPIC_F64 && prec != SPIC_F32)
+    return fail(SPIC_EINVAL, "precision must be SPIC_F64 or SPIC_F32");
+  if (e->grd.prec == SPIC_F64 && prec != SPIC_F64)
+    return fail(SPIC_EUNSUPPORTED, "an FP64 engine stores FP64 arrays only");
   if (is < 0 || is >= e->ns || !e->sp[is].loaded)
     return fail(SPIC_EINVAL, "species %d not loaded", is);
   int rc = spic::check_species(dst, true, false);
@@ -473,16 +546,20 @@ int spic_engine_store(spic_engine *e, int is, spic_species *dst, void *stream) {
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   void *dsts[7] = {dst->x, dst->y, dst->z, dst->u, dst->v, dst->w, dst->q};
+  const int f64 = prec == SPIC_F64 ? 1 : 0;
   hipError_t err =
       e->esz == 8
           ? spic::bin_store<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                    sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
-                                    (double *const *)dsts, s)
+                                    sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, dsts, f64, s)
           : spic::bin_store<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                   sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
-                                   (float *const *)dsts, s);
+                                   sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, dsts, f64, s);
   if (err == hipSuccess && !spic::async_mode()) err = hipStreamSynchronize(s);
   return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine store");
+}
+
+int spic_engine_store(spic_engine *e, int is, spic_species *dst, void *stream) {
+  if (!e) return fail(SPIC_EINVAL, "engine is NULL");
+  return spic_engine_store_as(e, is, dst, e->grd.prec, stream);
 }
 
 int spic_engine_snapshot_be32(spic_engine *e, int is, void *const dst[6], void *stream) {
@@ -533,6 +610,7 @@ int spic_engine_phase(spic_engine *e, const spic_field *field,
     if (!e->sp[is].loaded) continue;
     if ((rc = spic::check_moments(&mom[is]))) return rc;
   }
+  if ((rc = check_positions(e))) return rc;
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   hipError_t err;
@@ -559,6 +637,7 @@ int spic_engine_deposit(spic_engine *e, const spic_moments *mom, void *stream) {
     if (!e->sp[is].loaded) continue;
     if ((rc = spic::check_moments(&mom[is]))) return rc;
   }
+  if ((rc = check_positions(e))) return rc;
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   // neither is read: the kernels skip the push and the field stage
@@ -575,6 +654,11 @@ int spic_engine_deposit(spic_engine *e, const spic_moments *mom, void *stream) {
   return spic::end_call(s, st, err);
 }
 
+int spic_test_fail_engine_alloc(int after) {
+  g_fail_after.store(after < 0 ? -1 : after);
+  return SPIC_OK;
+}
+
 int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
   if (!e || !st) return fail(SPIC_EINVAL, "NULL argument");
   std::memset(st, 0, sizeof(*st));
@@ -585,6 +669,7 @@ int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
   st->tile_y = e->L.ty;
   st->tile_z = e->L.tz;
   st->device_bytes = e->bytes;
+  st->positions = e->L.fixed ? SPIC_POS_FIXED : SPIC_POS_FLOAT;
   unsigned long long c[4] = {0, 0, 0, 0};
   hipError_t err = hipMemcpy(c, e->counters, sizeof(c), hipMemcpyDeviceToHost);
   if (err != hipSuccess) return hip_fail(err, "engine counters");
@@ -746,6 +831,25 @@ hipError_t launch_batch_tile_sort(spic_batch *b, int state, hipStream_t s) {
                                   b->sort_ws_words, s);
 }
 
+// The argument tables hold only species with particles: a loaded species
+// without any deposits nothing, so with SPIC_BATCH_SET_MOMENTS its moment
+// set is zeroed here (fill kernels, which graph capture records as such).
+hipError_t zero_empty_batch_moments(const spic_batch *b, hipStream_t s) {
+  for (int i = 0; i < b->n; ++i) {
+    const spic_engine *e = b->e[i];
+    const size_t nn = (size_t)e->grd.nxn * e->grd.nyn * e->grd.nzn;
+    for (int is = 0; is < e->ns; ++is) {
+      if (!e->sp[is].loaded || e->sp[is].n != 0 || !b->mom[i][is].rhon) continue;
+      const spic::KMom m = spic::kmom(&b->mom[i][is]);
+      for (int c = 0; c < 10; ++c) {
+        const hipError_t err = spic::zero_f64(m.m[c], nn, s);
+        if (err != hipSuccess) return err;
+      }
+    }
+  }
+  return hipSuccess;
+}
+
 template <typename T>
 hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
   const spic::BinnedArgs<T> *tab =
@@ -753,12 +857,13 @@ hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
   const spic::BinLayout &L = b->e[0]->L;
   const int set = (b->flags & SPIC_BATCH_SET_MOMENTS) ? 1 : 0;
   hipError_t err = spic::launch_batch_zero_counts<T>(tab, b->counts, b->max_ns, set, s);
+  if (err == hipSuccess && set) err = zero_empty_batch_moments(b, s);
   if (err != hipSuccess) return err;
   err = spic::strict_mode()
             ? spic::spic_strict::launch_binned_phase_batch<T>(
-                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, s)
+                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, L.fixed, s)
             : spic::spic_fast::launch_binned_phase_batch<T>(
-                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, s);
+                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, L.fixed, s);
   if (err != hipSuccess) return err;
   return spic::launch_batch_reduce<T>(tab, b->nodes, b->max_ns, L.tx, L.ty, L.tz, s);
 }
@@ -785,10 +890,11 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
     if ((rc = spic::check_field(m.field)) || (rc = spic::check_push(m.push))) return rc;
     const spic_engine *e = m.engine, *e0 = members[0].engine;
     if (e->grd.prec != e0->grd.prec || e->L.tx != e0->L.tx || e->L.ty != e0->L.ty ||
-        e->L.tz != e0->L.tz || e->lds_fp64 != e0->lds_fp64 || e->sort_mode != e0->sort_mode)
+        e->L.tz != e0->L.tz || e->lds_fp64 != e0->lds_fp64 ||
+        e->sort_mode != e0->sort_mode || e->L.fixed != e0->L.fixed)
       return fail(SPIC_EINVAL,
-                  "batch member %d: precision, tile shape, LDS partials or sort mode "
-                  "differ from member 0's", i);
+                  "batch member %d: precision, tile shape, LDS partials, sort mode or "
+                  "position storage differ from member 0's", i);
     for (int j = 0; j < i; ++j)
       if (members[j].engine == e)
         return fail(SPIC_EINVAL, "batch members %d and %d share an engine", j, i);
@@ -826,6 +932,7 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
 
 int spic_batch_phase(spic_batch *b, void *stream) {
   if (!b) return fail(SPIC_EINVAL, "batch is NULL");
+  if (int prc = check_positions(b->e[0])) return prc;
   hipStream_t s = (hipStream_t)stream;
   hipError_t err;
   // every loaded species of the batch must be in one (layout, buffer) state
@@ -848,11 +955,16 @@ int spic_batch_phase(spic_batch *b, void *stream) {
                     "species, then advance them only through the batch", i, is);
     }
   }
-  if (state < 0) return SPIC_OK;  // nothing loaded
   const bool sort_now = b->sort_every > 0 && b->phases > 0 && b->phases % b->sort_every == 0;
   unsigned int *st;
   int rc;
   if ((rc = spic::begin_call(s, &st))) return rc;
+  if (state < 0) {
+    // no member holds a particle: the written moments are all zero
+    err = (b->flags & SPIC_BATCH_SET_MOMENTS) ? zero_empty_batch_moments(b, s) : hipSuccess;
+    if (err == hipSuccess) b->phases++;
+    return spic::end_call(s, st, err);
+  }
   if (stale || st != b->status) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if ((err = hipStreamIsCapturing(s, &cs)) != hipSuccess)

@@ -13,11 +13,16 @@
   same attributes, groups and datasets as a directory tree instead (one
   ``<dataset>.f32be`` file of big-endian float32 per dataset, the
   attributes in ``attrs.json``).
-* ``save_ic_data`` (src/IC.cpp:190-313): raw native arrays, one file each,
-  named ``<name>_<nxn>_<nyn>_<nzn>[_<is>]_<npmax>.bin``;
-  :func:`write_ic_particles` writes its per-species part (``rhon``,
-  ``rhoc``, ``x y z u v w q``).  The solver's auxiliary fields of that dump
-  (``Bxc``, ``Exth`` ...) belong to the field solver and are not written.
+* ``save_ic_data`` / ``read_ic_data`` (src/IC.cpp:190-313, 316-439): the
+  initial-condition dump a restart reads back, raw native arrays, one file
+  each: the node fields ``Bxn Byn Bzn``, the cell-centred ``Bxc Byc Bzc``,
+  the node fields ``Exn Eyn Ezn`` and ``Exth Eyth Ezth``
+  (``<name>_<n0>_<n1>_<n2>.bin``), then per species ``rhon`` and ``rhoc``
+  (``<name>_<n0>_<n1>_<n2>_<npmax>.bin``, no species index: in the reference
+  two species with the same npmax share these two files, the last one
+  written wins) and ``x y z u v w q`` (``<name>_<nxn>_<nyn>_<nzn>_<is>_<npmax>.bin``).
+  :func:`write_ic_data` writes the whole dump and :func:`read_ic_data`
+  reads it; :func:`write_ic_particles` writes one species' files.
 """
 from __future__ import annotations
 
@@ -165,14 +170,114 @@ def write_h5(path: str, cycle: int, species: Sequence[Dict[str, np.ndarray]],
     return path
 
 
-def ic_file_name(kind: str, shape, npmax: int, species: Optional[int] = None) -> str:
-    """save_ic_data's file names (src/IC.cpp:259-313): densities
-    <kind>_<n0>_<n1>_<n2>_<npmax>.bin, particle arrays
-    <kind>_<nxn>_<nyn>_<nzn>_<is>_<npmax>.bin."""
+def ic_file_name(kind: str, shape, npmax: Optional[int] = None,
+                 species: Optional[int] = None) -> str:
+    """save_ic_data's file names (src/IC.cpp:195-313): fields
+    <kind>_<n0>_<n1>_<n2>.bin, densities <kind>_<n0>_<n1>_<n2>_<npmax>.bin,
+    particle arrays <kind>_<nxn>_<nyn>_<nzn>_<is>_<npmax>.bin."""
     dims = "_".join(str(int(v)) for v in shape)
+    if npmax is None:
+        return f"{kind}_{dims}.bin"
     if species is None:
         return f"{kind}_{dims}_{int(npmax)}.bin"
     return f"{kind}_{dims}_{int(species)}_{int(npmax)}.bin"
+
+
+# save_ic_data's field files in its order: (file kind, key in the caller's
+# dicts, on cells)
+IC_FIELDS = (("Bxn", "Bxn", False), ("Byn", "Byn", False), ("Bzn", "Bzn", False),
+             ("Bxc", "Bxc", True), ("Byc", "Byc", True), ("Bzc", "Bzc", True),
+             ("Exn", "Ex", False), ("Eyn", "Ey", False), ("Ezn", "Ez", False),
+             ("Exth", "Exth", False), ("Eyth", "Eyth", False), ("Ezth", "Ezth", False))
+IC_SPECIES = ("x", "y", "z", "u", "v", "w", "q")
+
+
+def interp_n2c_field(grd, bn: np.ndarray) -> np.ndarray:
+    """interpN2Cfield (src/Grid.cpp:86-120) for one component: the mean of
+    the eight nodes around each interior cell, .125 times their sum in the
+    reference's order; guard cells zero.  initGEM makes the cell-centred B
+    this way (src/IC.cpp:99-100)."""
+    b = np.asarray(bn).reshape(grd.nxn, grd.nyn, grd.nzn)
+    out = np.zeros((grd.nxc, grd.nyc, grd.nzc), dtype=b.dtype)
+    i = slice(1, grd.nxc - 1)
+    j = slice(1, grd.nyc - 1)
+    k = slice(1, grd.nzc - 1)
+    i1 = slice(2, grd.nxc)
+    j1 = slice(2, grd.nyc)
+    k1 = slice(2, grd.nzc)
+    s = b[i, j, k] + b[i1, j, k]
+    s = s + b[i, j1, k]
+    s = s + b[i, j, k1]
+    s = s + b[i1, j1, k]
+    s = s + b[i1, j, k1]
+    s = s + b[i, j1, k1]
+    s = s + b[i1, j1, k1]
+    out[i, j, k] = (0.125 * s.astype(np.float64)).astype(b.dtype)
+    return out.reshape(-1)
+
+
+def write_ic_data(restart_dir: str, grd, fields: Dict[str, np.ndarray],
+                  species: Sequence[Dict[str, np.ndarray]], field_dtype, part_dtype,
+                  interp_dtype, aux: Optional[Dict[str, np.ndarray]] = None) -> List[str]:
+    """save_ic_data (src/IC.cpp:190-313): every file of the dump into
+    restart_dir, in the reference's order, names and native types.
+
+    fields: Ex Ey Ez Bxn Byn Bzn on the nodes.  aux: Bxc Byc Bzc (cells)
+    and Exth Eyth Ezth (nodes); when omitted, as initGEM leaves them: the
+    cell-centred B from interp_n2c_field and E theta zero.  species: per
+    species a dict with rhon (nodes), rhoc (cells) and x y z u v w q (q in
+    the FPpart type, as the reference writes it)."""
+    os.makedirs(restart_dir, exist_ok=True)
+    nodes = (grd.nxn, grd.nyn, grd.nzn)
+    cells = (grd.nxc, grd.nyc, grd.nzc)
+    if aux is None:
+        aux = {"Bxc": interp_n2c_field(grd, np.asarray(fields["Bxn"], field_dtype)),
+               "Byc": interp_n2c_field(grd, np.asarray(fields["Byn"], field_dtype)),
+               "Bzc": interp_n2c_field(grd, np.asarray(fields["Bzn"], field_dtype))}
+        for n in ("Exth", "Eyth", "Ezth"):
+            aux[n] = np.zeros(grd.nxn * grd.nyn * grd.nzn, field_dtype)
+    src = dict(fields)
+    src.update(aux)
+    written = []
+    for kind, key, on_cells in IC_FIELDS:
+        p = os.path.join(restart_dir, ic_file_name(kind, cells if on_cells else nodes))
+        np.ascontiguousarray(src[key], dtype=field_dtype).tofile(p)
+        written.append(p)
+    for s, sp in enumerate(species):
+        written += write_ic_particles(restart_dir, grd, s, sp["rhon"], sp["rhoc"], sp,
+                                      part_dtype, interp_dtype)
+    return written
+
+
+def read_ic_data(restart_dir: str, grd, npmax: Sequence[int], field_dtype, part_dtype,
+                 interp_dtype):
+    """read_ic_data (src/IC.cpp:316-439): the dump's files back, as
+    (fields, aux, species) in the shapes write_ic_data takes them; npmax per
+    species names the files (and sizes the particle arrays)."""
+    nodes = (grd.nxn, grd.nyn, grd.nzn)
+    cells = (grd.nxc, grd.nyc, grd.nzc)
+    nn = grd.nxn * grd.nyn * grd.nzn
+    nc = grd.nxc * grd.nyc * grd.nzc
+
+    def load(name, dtype, count):
+        a = np.fromfile(os.path.join(restart_dir, name), dtype=dtype)
+        if a.size != count:
+            raise ValueError(f"{name}: {a.size} values, {count} expected")
+        return a
+
+    fields, aux = {}, {}
+    for kind, key, on_cells in IC_FIELDS:
+        a = load(ic_file_name(kind, cells if on_cells else nodes), field_dtype,
+                 nc if on_cells else nn)
+        (aux if key in ("Bxc", "Byc", "Bzc", "Exth", "Eyth", "Ezth") else fields)[key] = a
+    species = []
+    for s, n in enumerate(npmax):
+        sp = {"rhon": load(ic_file_name("rhon", nodes, n), interp_dtype, nn),
+              "rhoc": load(ic_file_name("rhoc", cells, n), interp_dtype, nc)}
+        for k in IC_SPECIES:
+            sp[k] = load(ic_file_name(k, nodes, n, s), part_dtype, n)
+        species.append(sp)
+    return fields, aux, species
 
 
 def write_ic_particles(restart_dir: str, grd, species: int, rhon: np.ndarray,
@@ -189,7 +294,7 @@ def write_ic_particles(restart_dir: str, grd, species: int, rhon: np.ndarray,
         p = os.path.join(restart_dir, ic_file_name(kind, shape, npmax))
         np.ascontiguousarray(arr, dtype=interp_dtype).tofile(p)
         written.append(p)
-    for kind in ("x", "y", "z", "u", "v", "w", "q"):
+    for kind in IC_SPECIES:
         p = os.path.join(restart_dir, ic_file_name(kind, nodes, npmax, species))
         np.ascontiguousarray(parts[kind], dtype=part_dtype).tofile(p)
         written.append(p)

@@ -43,7 +43,7 @@ def oracle_small(prec):
 
 
 def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0,
-               slack_pct=0, set_moments=False, sort_mode=0):
+               slack_pct=0, set_moments=False, sort_mode=0, positions=0):
     c = st.cfg
     grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, prec)
     dt = np.float64 if prec == "f64" else np.float32
@@ -53,7 +53,7 @@ def run_engine(st: O.State, prec: str, phases: int, tile=(4, 4, 4), sort_every=0
     param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover, PERIODICX=c.PERIODICX,
                          PERIODICY=c.PERIODICY, PERIODICZ=c.PERIODICZ)
     eng = S.Engine(grd, c.ns, tile=tile, sort_every=sort_every, slack_pct=slack_pct,
-                   set_moments=set_moments, sort_mode=sort_mode)
+                   set_moments=set_moments, sort_mode=sort_mode, positions=positions)
     for s, p in enumerate(parts):
         eng.load(s, p)
     if set_moments:
@@ -349,14 +349,80 @@ def test_engine_overflow_region(prec, set_moments):
                          f"s{s} {n}")
 
 
-@pytest.mark.parametrize("prec", PRECS)
-def test_engine_load_store_round_trip(prec):
+@pytest.mark.parametrize("prec,positions", [("f64", 0), ("f32", S.SPIC_POS_FLOAT),
+                                            ("f32", S.SPIC_POS_FIXED)])
+def test_engine_load_store_round_trip(prec, positions):
+    """Load then store without a phase returns every particle at its load
+    index: bit for bit with float positions (and for velocities and
+    charges always); fixed-point positions come back within half their
+    unit (2^-(k+1) cells) rounded to float."""
     st = oracle_small(prec)
-    got_p, _, stats = run_engine(st, prec, phases=0)
+    got_p, _, stats = run_engine(st, prec, phases=0, positions=positions)
     assert stats["sorts"] == st.cfg.ns and stats["overflow"] == 0
-    for s in range(st.cfg.ns):
+    assert stats["positions"] == (positions or S.SPIC_POS_FLOAT)
+    c = st.cfg
+    for s in range(c.ns):
         for k in O.PARTICLE_NAMES:
-            assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+            if positions == S.SPIC_POS_FIXED and k in ("x", "y", "z"):
+                L, n = {"x": (c.Lx, c.nxc), "y": (c.Ly, c.nyc), "z": (c.Lz, c.nzc)}[k]
+                kbits = max(b for b in range(25) if (n + 64) << b < 2 ** 31)
+                want = st.parts[s][k].astype(np.float64)
+                err = np.abs(got_p[s][k].astype(np.float64) - want)
+                bound = 0.5 * (L / n) * 2.0 ** -kbits + np.spacing(np.abs(want).astype(
+                    np.float32)).astype(np.float64)
+                assert np.all(err <= bound), f"s{s} {k}: {np.max(err - bound)}"
+            else:
+                assert_bitwise(got_p[s][k], st.parts[s][k], f"s{s} {k}")
+
+
+def test_engine_fp32_fixed_point_keeps_fp64_input_precision():
+    """An FP32 engine with fixed-point positions loaded from FP64 arrays
+    (spic_engine_load_as) and stored into FP64 arrays returns positions
+    within half the fixed-point unit of the FP64 input -- about 1e-8 of a
+    box length here, 100 times finer than float lengths near the far faces
+    -- and velocities and charges rounded to float."""
+    st = oracle_small("f64")
+    c = st.cfg
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f32")
+    eng = S.Engine(grd, c.ns, positions=S.SPIC_POS_FIXED)
+    try:
+        for s in range(c.ns):
+            p64 = S.Particles(st.parts[s], c.qom[s], "f64")
+            eng.load(s, p64)
+            out = S.DeviceParticles(len(st.parts[s]["x"]), c.qom[s], "f64")
+            eng.store(s, out)
+            got = out.download()
+            for k, L, n in (("x", c.Lx, c.nxc), ("y", c.Ly, c.nyc), ("z", c.Lz, c.nzc)):
+                kbits = max(b for b in range(25) if (n + 64) << b < 2 ** 31)
+                err = np.max(np.abs(got[k] - st.parts[s][k]))
+                assert err <= 0.5 * (L / n) * 2.0 ** -kbits * (1 + 1e-9), (s, k, err)
+            for k in ("u", "v", "w", "q"):
+                assert_bitwise(got[k], st.parts[s][k].astype(np.float32).astype(np.float64),
+                               f"s{s} {k}")
+    finally:
+        eng.close()
+
+
+def test_engine_fixed_point_refused_in_strict_arithmetic():
+    st = oracle_small("f32")
+    c = st.cfg
+    grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f32")
+    eng = S.Engine(grd, 1)  # created in production arithmetic: fixed point
+    try:
+        eng.load(0, S.Particles(st.parts[0], c.qom[0], "f32"))
+        assert eng.stats()["positions"] == S.SPIC_POS_FIXED
+        field = S.EMField(grd, {n: st.fields[n] for n in O.FIELD_NAMES})
+        S.set_strict_fp(True)
+        with pytest.raises(S.SpicError) as ei:
+            eng.phase(field, [S.InterpDensSpecies(grd)], S.Parameters())
+        assert ei.value.code == S.SPIC_EINVAL
+    finally:
+        S.set_strict_fp(False)
+        eng.close()
+    with pytest.raises(S.SpicError) as ei:
+        S.Engine(S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f64"), 1,
+                 positions=S.SPIC_POS_FIXED)
+    assert ei.value.code == S.SPIC_EUNSUPPORTED
 
 
 @pytest.mark.parametrize("prec", PRECS)

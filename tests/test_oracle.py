@@ -140,6 +140,44 @@ def test_oracle_matches_live_reference_full_small_case(prec):
     ref.close()
 
 
+@pytest.mark.parametrize("prec", ["f64", "f64_O3"])
+def test_reference_state_from_external_inputs_matches_oracle(prec):
+    """The benchmark's CPU reference leg builds the reference's state
+    without initGEM and copies in inputs made elsewhere (the device
+    generator's, here perturbed fields and particles that wrap and reflect):
+    two cycles of the reference's own mover_PC + interpP2G, at its timing
+    build's -O3 too, equal the oracle's bit for bit."""
+    if not O.ref_available(prec):
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    cfg = O.GemConfig(nxc=8, nyc=6, nzc=4, Lx=2.0, Ly=1.5, Lz=1.0, npcelx=[2] * 4,
+                      npcely=[2] * 4, npcelz=[2] * 4, dt=0.4, uth=[0.1] * 4, vth=[0.1] * 4,
+                      wth=[0.1] * 4)
+    st = O.State(cfg, "f64")
+    rng = np.random.default_rng(8)
+    for n in O.FIELD_NAMES:
+        st.fields[n] += rng.normal(0, 1e-4, st.nn)
+    # at most 5.3 thermal speeds: under one cell (0.25) per cycle, so that
+    # the reference, which has no bounds checks, stays inside its arrays
+    for p in st.parts:
+        assert max(np.abs(p[k]).max() for k in "uvw") * cfg.dt < 0.25
+    ref = O.RefState(cfg, prec, ic=False)
+    for s in range(cfg.ns):
+        assert ref.nop(s) == len(st.parts[s]["x"])
+        ref.set_particles(s, st.parts[s])
+    for w, n in enumerate(O.FIELD_NAMES):
+        ref.set_field(w, st.fields[n])
+    for _ in range(2):
+        ref.time_cycles(1)
+        st.cycle()
+    for s in range(cfg.ns):
+        rp = ref.particles(s)
+        for k in O.PARTICLE_NAMES:
+            assert_bitwise(st.parts[s][k], rp[k], f"s{s} {k}")
+        for w, n in enumerate(O.MOMENT_NAMES):
+            assert_bitwise(st.moments[s][n], ref.moment(s, w), f"s{s} {n}")
+    ref.close()
+
+
 # --------------------------------------------------------------------------
 # Closed forms for the mover.
 # --------------------------------------------------------------------------
