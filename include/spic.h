@@ -308,6 +308,8 @@ typedef struct spic_engine_stats {
   int64_t lds_deposits, global_deposits;
   int64_t overflow;     /* particles now in overflow regions (all species) */
   int positions;        /* SPIC_POS_FLOAT or SPIC_POS_FIXED */
+  size_t workspace_bytes; /* of device_bytes: the sort workspace, kept
+                             between sorts and grown on demand */
 } spic_engine_stats;
 
 /* Create an engine for `ns` species (1..8) on `grd` (its node tables are

@@ -107,7 +107,8 @@ class spic_engine_stats(C.Structure):
                 ("device_bytes", C.c_size_t),
                 ("lds_gathers", C.c_int64), ("global_gathers", C.c_int64),
                 ("lds_deposits", C.c_int64), ("global_deposits", C.c_int64),
-                ("overflow", C.c_int64), ("positions", C.c_int)]
+                ("overflow", C.c_int64), ("positions", C.c_int),
+                ("workspace_bytes", C.c_size_t)]
 
 
 class spic_batch_member(C.Structure):

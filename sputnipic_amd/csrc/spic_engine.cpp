@@ -670,6 +670,7 @@ int spic_engine_get_stats(const spic_engine *e, spic_engine_stats *st) {
   st->tile_z = e->L.tz;
   st->device_bytes = e->bytes;
   st->positions = e->L.fixed ? SPIC_POS_FIXED : SPIC_POS_FLOAT;
+  st->workspace_bytes = e->ws_bytes;
   unsigned long long c[4] = {0, 0, 0, 0};
   hipError_t err = hipMemcpy(c, e->counters, sizeof(c), hipMemcpyDeviceToHost);
   if (err != hipSuccess) return hip_fail(err, "engine counters");
