@@ -142,6 +142,19 @@ int spic_mover_interp(const spic_grid *grd, const spic_field *field,
 int spic_particle_sort(const spic_grid *grd, int64_t sort_cps,
                        spic_species *sp, void *stream);
 
+/* particle_sort without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, result and errors as spic_particle_sort; the sort's scratch
+ * is a fixed device array of SPIC_SORT_NOALLOC_SCRATCH_MIB MiB per GPU,
+ * held from the first call on that GPU until the process ends, which
+ * covers about 2.2M particles of FP64 (2.7M FP32); a larger npmax is
+ * refused with SPIC_EINVAL.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+#define SPIC_SORT_NOALLOC_SCRATCH_MIB 64
+int spic_particle_sort_noalloc(const spic_grid *grd, int64_t sort_cps,
+                               spic_species *sp, void *stream);
+
 /* The same snapshot words for a species in the caller's order (its nop
  * particles, element type per grd->prec). */
 int spic_snapshot_be32(const spic_grid *grd, const spic_species *sp,

@@ -127,6 +127,8 @@ SIGNATURES = {
                                     C.POINTER(spic_push), _P]),
     "spic_particle_sort": (C.c_int, [C.POINTER(spic_grid), C.c_int64,
                                      C.POINTER(spic_species), _P]),
+    "spic_particle_sort_noalloc": (C.c_int, [C.POINTER(spic_grid), C.c_int64,
+                                             C.POINTER(spic_species), _P]),
     "spic_zero_moments": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments), _P]),
     "spic_apply_bc_ids": (C.c_int, [C.POINTER(spic_grid), C.POINTER(spic_moments), C.c_int,
                                     C.POINTER(spic_push), _P]),

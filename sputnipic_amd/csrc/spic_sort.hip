@@ -8,6 +8,7 @@
 // seven arrays are then gathered through it one at a time into one scratch
 // array and copied back.
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -159,3 +160,73 @@ template hipError_t sort_reference_order<float>(const spic_grid &, int64_t,
                                                 size_t *, hipStream_t);
 
 }  // namespace spic
+
+// ---- particle_sort without allocation ------------------------------------
+//
+// The sort's scratch (bin keys in and out, indices, the permutation, the
+// gather staging array and rocPRIM's temporary storage) in a fixed device
+// array, one copy per GPU, held until the process ends.  Nothing in it has
+// to start at zero: every part is written before it is read within a call.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(SPIC_SORT_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device, looked up once per device with
+// that device current.  The host-side name `scratch` is not a device address
+// and is never passed to a kernel or a copy.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_pointer(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return spic::hip_fail(e, "particle_sort_noalloc: current device");
+  if (dev < 0 || dev >= kMaxDevices)
+    return spic::fail(SPIC_EUNSUPPORTED, "particle_sort_noalloc: device %d beyond %d", dev,
+                      kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return spic::hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "particle_sort_noalloc: scratch address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SPIC_OK;
+}
+
+}  // namespace
+
+extern "C" int spic_particle_sort_noalloc(const spic_grid *grd, int64_t sort_cps,
+                                          spic_species *sp, void *stream) {
+  int rc;
+  if ((rc = spic::check_grid(grd)) || (rc = spic::check_species(sp, true, false))) return rc;
+  if (sort_cps < 1) return spic::fail(SPIC_EINVAL, "sort_cps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  size_t need = 0;
+  hipError_t e =
+      grd->prec == SPIC_F64
+          ? spic::sort_reference_order<double>(*grd, sort_cps, *sp, nullptr, 0, &need, s)
+          : spic::sort_reference_order<float>(*grd, sort_cps, *sp, nullptr, 0, &need, s);
+  if (e != hipSuccess) return spic::hip_fail(e, "particle_sort_noalloc: sizing");
+  if (need > kNoAllocScratchBytes)
+    return spic::fail(SPIC_EINVAL,
+                      "particle_sort_noalloc: %lld particles need %zu bytes of scratch; the "
+                      "fixed scratch holds %zu (use spic_particle_sort)",
+                      (long long)sp->npmax, need, kNoAllocScratchBytes);
+  void *ws = nullptr;
+  if ((rc = scratch_pointer(&ws))) return rc;
+  e = grd->prec == SPIC_F64
+          ? spic::sort_reference_order<double>(*grd, sort_cps, *sp, ws, need, nullptr, s)
+          : spic::sort_reference_order<float>(*grd, sort_cps, *sp, ws, need, nullptr, s);
+  if (e != hipSuccess) return spic::hip_fail(e, "particle_sort_noalloc");
+  if (!spic::async_mode()) {
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return spic::hip_fail(e, "particle_sort_noalloc: execution");
+  }
+  return SPIC_OK;
+}
