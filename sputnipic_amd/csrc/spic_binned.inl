@@ -103,17 +103,14 @@ __device__ __forceinline__ T bdenom(T omdtsq) {
 // arithmetic).  The mover iterate's cell truncates the cell coordinate
 // toward zero as the reference's int() does (src/Mover.cpp:45-47); the
 // deposit cell is its floor (src/Mover.cpp:411-413).
-__device__ __forceinline__ int fx_trunc_cell(int X, int k) {
-  return X >= 0 ? (X >> k) : -((-X) >> k);
-}
-
-// A displacement d (fixed units) rounded to an integer, bounded to 32
-// cells: with k chosen so that (cells + 64) << k fits in int32, the sum
-// with a coordinate in the box plus its excursion cannot overflow; a
-// particle that moves farther leaves the grid and is reported.
-__device__ __forceinline__ int fx_step(float d, int k) {
-  const float lim = (float)(32 << k);
-  return __float2int_rn(fminf(fmaxf(d, -lim), lim));
+//
+// A displacement d (fixed units) rounded to an integer, bounded to `lim`
+// (64 cells): with k chosen so that (cells + 64) << k fits in int32, the sum
+// with a coordinate in the box cannot overflow.  A particle whose final
+// displacement exceeds the bound (or is not a number) is reported by the
+// caller; one that moves that far also leaves any grid the bound allows.
+__device__ __forceinline__ int fx_step(float d, float lim) {
+  return __float2int_rn(__builtin_amdgcn_fmed3f(d, -lim, lim));
 }
 
 // Offset of X from the left node of cell c, in cells: exact while X lies
@@ -296,37 +293,72 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
 // fixed-point unit, and the boundary conditions applied exactly on the
 // integers: x > Lx is X > lim, a reflection lim - (X - lim), a wrap
 // X - lim.
+//
+// The iterate's cell truncates toward zero as the reference's int() does
+// (src/Mover.cpp:45-47): an arithmetic shift (the floor), corrected for
+// negative coordinates only in waves that hold one.  An iterate whose
+// stencil lies in the staged block (lim_b: the last valid lower-node index
+// per axis, the block's edge or the grid's) needs no clamp: every index is
+// valid.  Any other takes the clamped, flagged indices of bclamp and
+// gathers from global memory.  The cached stencil is keyed by its lower
+// block node (staged) or by its packed global cell (not staged).
 template <bool STAGED, int TX, int TY, int TZ>
 __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<float> &f,
                                            const KPush<float> &k, const FixedParams &fp,
-                                           const Block<float, TX, TY, TZ> &blk, int &X,
-                                           int &Y, int &Z, float &u, float &v, float &w,
-                                           bool &bad, unsigned int &n_lds,
-                                           unsigned int &n_glob) {
+                                           const Block<float, TX, TY, TZ> &blk,
+                                           const int lim_b[3], int &X, int &Y, int &Z,
+                                           float &u, float &v, float &w, bool &bad,
+                                           unsigned int &n_lds, unsigned int &n_glob) {
   using B = Block<float, TX, TY, TZ>;
   const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
   const int Xt = X, Yt = Y, Zt = Z;
   int Xc = X, Yc = Y, Zc = Z;
   float upt = 0.f, vpt = 0.f, wpt = 0.f;
-  int cix = -100, ciy = -100, ciz = -100;  // cell of the cached stencil
+  int ckey = 0x7fffffff;  // key of the cached stencil (none yet)
   float F[2][2][2][6];
-  // fixed units per unit velocity over half a step
+  const int kx = fp.k[0], ky = fp.k[1], kz = fp.k[2];
+  const int mx = (1 << kx) - 1, my = (1 << ky) - 1, mz = (1 << kz) - 1;
+  // fixed units per unit velocity over half a step and over the step; a
+  // displacement is bounded to 64 cells, which keeps the sum with any
+  // coordinate of the box in int32 (fixed_params leaves 64 cells of room)
   const float hx = k.dto2 * fp.s[0], hy = k.dto2 * fp.s[1], hz = k.dto2 * fp.s[2];
+  const float bx_lim = (float)(64 << kx), by_lim = (float)(64 << ky), bz_lim = (float)(64 << kz);
   for (int it = 0; it < k.niter; ++it) {
-    int ix = 2 + fx_trunc_cell(Xc, fp.k[0]);
-    int iy = 2 + fx_trunc_cell(Yc, fp.k[1]);
-    int iz = 2 + fx_trunc_cell(Zc, fp.k[2]);
-    bad |= bclamp(ix, g.nxn);
-    bad |= bclamp(iy, g.nyn);
-    bad |= bclamp(iz, g.nzn);
-    if (ix != cix || iy != ciy || iz != ciz) {
-      cix = ix; ciy = iy; ciz = iz;
-      const int bx = ix - 1 - blk.c0x, by = iy - 1 - blk.c0y, bz = iz - 1 - blk.c0z;
-      const bool in_block = STAGED && (bx >= 0) & (bx <= B::BX - 2) & (by >= 0) &
-                                          (by <= B::BY - 2) & (bz >= 0) & (bz <= B::BZ - 2);
+    // physical cell (floor) and offset within it, in fixed units
+    int cx = Xc >> kx, cy = Yc >> ky, cz = Zc >> kz;
+    int ox_i = Xc & mx, oy_i = Yc & my, oz_i = Zc & mz;
+    if (__builtin_expect(__any(min(Xc, min(Yc, Zc)) < 0), 0)) {
+      // truncation toward zero: -((-X) >> k), offset -((-X) & mask)
+      if (Xc < 0) { cx = -((-Xc) >> kx); ox_i = -((-Xc) & mx); }
+      if (Yc < 0) { cy = -((-Yc) >> ky); oy_i = -((-Yc) & my); }
+      if (Zc < 0) { cz = -((-Zc) >> kz); oz_i = -((-Zc) & mz); }
+    }
+    // block index of the stencil's lower node (global node cx + 1)
+    const int bx = cx + 1 - blk.c0x, by = cy + 1 - blk.c0y, bz = cz + 1 - blk.c0z;
+    const bool in_block = STAGED && ((unsigned)bx <= (unsigned)lim_b[0]) &
+                                        ((unsigned)by <= (unsigned)lim_b[1]) &
+                                        ((unsigned)bz <= (unsigned)lim_b[2]);
+    int key, ix = 0, iy = 0, iz = 0;
+    if (in_block) {
+      key = (bx * B::BY + by) * B::BZ + bz;
+    } else {
+      ix = cx + 2; iy = cy + 2; iz = cz + 2;
+      bad |= bclamp(ix, g.nxn);
+      bad |= bclamp(iy, g.nyn);
+      bad |= bclamp(iz, g.nzn);
+      // offsets from the (clamped) cell's left node, extrapolated beyond it
+      ox_i = Xc - ((ix - 2) << kx);
+      oy_i = Yc - ((iy - 2) << ky);
+      oz_i = Zc - ((iz - 2) << kz);
+      key = ~((ix << 20) | (iy << 10) | iz);
+    }
+    if (key != ckey) {
+      ckey = key;
       if (in_block) {
         ++n_lds;
-        const float *fb = blk.fld + (((bx + 1) * B::BY + (by + 1)) * B::BZ + (bz + 1)) * 6;
+        // node (bx + 1 - aa, by + 1 - bb, bz + 1 - cc) lies a compile-time
+        // offset before the upper corner (bx + 1, by + 1, bz + 1)
+        const float *fb = blk.fld + (key + (B::BY + 1) * B::BZ + 1) * 6;
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
@@ -360,9 +392,9 @@ __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<f
             }
       }
     }
-    const float ox = fx_frac(Xc, ix - 2, fp.k[0], fp.unit[0]);
-    const float oy = fx_frac(Yc, iy - 2, fp.k[1], fp.unit[1]);
-    const float oz = fx_frac(Zc, iz - 2, fp.k[2], fp.unit[2]);
+    const float ox = (float)ox_i * fp.unit[0];
+    const float oy = (float)oy_i * fp.unit[1];
+    const float oz = (float)oz_i * fp.unit[2];
     const float xi[2] = {ox, 1.f - ox};
     const float eta[2] = {oy, 1.f - oy};
     const float zeta[2] = {oz, 1.f - oz};
@@ -391,16 +423,19 @@ __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<f
     upt = (ut + q2 * (vt * Bzl - wt * Byl + q2 * udotb * Bxl)) * denom;
     vpt = (vt + q2 * (wt * Bxl - ut * Bzl + q2 * udotb * Byl)) * denom;
     wpt = (wt + q2 * (ut * Byl - vt * Bxl + q2 * udotb * Bzl)) * denom;
-    Xc = Xt + fx_step(upt * hx, fp.k[0]);
-    Yc = Yt + fx_step(vpt * hy, fp.k[1]);
-    Zc = Zt + fx_step(wpt * hz, fp.k[2]);
+    Xc = Xt + fx_step(upt * hx, bx_lim);
+    Yc = Yt + fx_step(vpt * hy, by_lim);
+    Zc = Zt + fx_step(wpt * hz, bz_lim);
   }
   u = 2.f * upt - u;
   v = 2.f * vpt - v;
   w = 2.f * wpt - w;
-  X = Xt + fx_step(upt * (k.dt_sub * fp.s[0]), fp.k[0]);
-  Y = Yt + fx_step(vpt * (k.dt_sub * fp.s[1]), fp.k[1]);
-  Z = Zt + fx_step(wpt * (k.dt_sub * fp.s[2]), fp.k[2]);
+  const float dX = upt * (k.dt_sub * fp.s[0]), dY = vpt * (k.dt_sub * fp.s[1]),
+              dZ = wpt * (k.dt_sub * fp.s[2]);
+  bad |= !((fabsf(dX) <= bx_lim) & (fabsf(dY) <= by_lim) & (fabsf(dZ) <= bz_lim));
+  X = Xt + fx_step(dX, bx_lim);
+  Y = Yt + fx_step(dY, by_lim);
+  Z = Zt + fx_step(dZ, bz_lim);
   // particle boundary conditions, two independent tests per axis
   // (src/Mover.cpp:104-150), exact on the integers
   if (X > fp.lim[0]) {
@@ -428,7 +463,6 @@ __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<f
     else { w = -w; Z = -Z; }
   }
 }
-
 // Tile of the deposit cell (ix, iy, iz), clamped to the physical cells;
 // the tile shape is a compile-time constant (divisions become shifts).
 template <int TX, int TY, int TZ, typename T>
@@ -439,56 +473,91 @@ __device__ __forceinline__ int tile_of(const BinnedArgs<T> &a, int ix, int iy, i
   return ((pcx / TX) * a.nty + pcy / TY) * a.ntz + pcz / TZ;
 }
 
-// Output slot of a particle to which its destination tile's counter gave
-// `slot`: in the segment while it has room, else the next slot of the
-// overflow region; -1 if that is full too.
+// Slots in the phase kernels are 32-bit: the host keeps every component
+// array of a species below 4 GiB (spic_engine_load), so a slot's byte
+// offset fits in 32 bits and each load or store is one uniform component
+// base plus one per-lane offset.  kNoSlot marks "no slot".
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+// The particles of range t (see RangeView) with 32-bit slots.
+struct SlotRange {
+  uint32_t first, count, gap_lo, gap_len;
+  __device__ uint32_t slot(uint32_t r) const { return first + (r < gap_lo ? r : r + gap_len); }
+};
+
 template <typename T>
-__device__ __forceinline__ int64_t out_slot(const BinnedArgs<T> &a,
-                                            const BinnedSpecies<T> &sp, int dest,
-                                            unsigned int slot) {
-  const int64_t sb = gmem(sp.seg)[dest];
-  if ((int64_t)slot < gmem(sp.seg)[dest + 1] - sb) return sb + slot;
-  const unsigned int k = gatomic_add(sp.cnt_out + a.ntiles, 1u);
-  return k < sp.ovf_cap ? gmem(sp.seg)[a.ntiles] + (int64_t)k : -1;
+__device__ __forceinline__ SlotRange slot_range(const BinnedArgs<T> &a,
+                                                const BinnedSpecies<T> &sp, int t) {
+  const RangeView v = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, t);
+  return SlotRange{(uint32_t)v.first, (uint32_t)v.count, v.gap_lo, v.gap_len};
+}
+
+// Component c (x y z u v w q) of the slot at byte offset `off`.
+template <typename T>
+__device__ __forceinline__ T load_comp(const BinnedSpecies<T> &sp, int c, uint32_t off) {
+  const SPIC_GLOBAL char *base =
+      (const SPIC_GLOBAL char *)gmem(sp.in) + (size_t)c * (size_t)sp.stride * sizeof(T);
+  return *(const SPIC_GLOBAL T *)(base + off);
 }
 
 template <typename T>
-__device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, int64_t o,
+__device__ __forceinline__ void store_comp(const BinnedSpecies<T> &sp, int c, uint32_t off,
+                                           T v) {
+  SPIC_GLOBAL char *base =
+      (SPIC_GLOBAL char *)gmem(sp.out) + (size_t)c * (size_t)sp.stride * sizeof(T);
+  *(SPIC_GLOBAL T *)(base + off) = v;
+}
+
+// Slot of a particle to which segment `dest`'s counter gave `k`: in the
+// segment while it has room, else the next slot of the overflow region;
+// kNoSlot if that is full too.
+template <typename T>
+__device__ __forceinline__ uint32_t seg_slot(const BinnedArgs<T> &a,
+                                             const BinnedSpecies<T> &sp, int dest,
+                                             unsigned int k) {
+  const uint32_t lo = (uint32_t)gmem(sp.seg)[dest];
+  const uint32_t cap = (uint32_t)gmem(sp.seg)[dest + 1] - lo;
+  if (k < cap) return lo + k;
+  const unsigned int o = gatomic_add(sp.cnt_out + a.ntiles, 1u);
+  return o < sp.ovf_cap ? (uint32_t)gmem(sp.seg)[a.ntiles] + o : kNoSlot;
+}
+
+template <typename T>
+__device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, uint32_t o,
                                                T x, T y, T z, T u, T v, T w, T q,
                                                uint32_t pid) {
-  SPIC_GLOBAL T *p = gmem(sp.out) + o;
-  const int64_t st = sp.stride;
-  p[0] = x; p[st] = y; p[2 * st] = z;
-  p[3 * st] = u; p[4 * st] = v; p[5 * st] = w;
-  p[6 * st] = q;
-  gmem(sp.id_out)[o] = pid;
+  const uint32_t off = o * (uint32_t)sizeof(T);
+  store_comp(sp, 0, off, x); store_comp(sp, 1, off, y); store_comp(sp, 2, off, z);
+  store_comp(sp, 3, off, u); store_comp(sp, 4, off, v); store_comp(sp, 5, off, w);
+  store_comp(sp, 6, off, q);
+  *(SPIC_GLOBAL uint32_t *)((SPIC_GLOBAL char *)gmem(sp.id_out) + o * 4u) = pid;
 }
 
 // Write a pushed particle to the output buffer.  A particle that stays in
-// `tile` takes slot j of the block the workgroup reserved in its own
-// segment (from stay_base); a leaver takes the next slot of its destination
-// segment's counter; when the segment is full, the next slot of the overflow
-// region.  False if that is full too (the particle is lost and the call
-// reports it).
+// its tile takes slot j of the block the workgroup reserved in its own
+// segment (own_lo + stay_base + j while that is below own_cap); a leaver
+// takes the next slot of its destination segment's counter; when the
+// segment is full, the next slot of the overflow region.  False if that is
+// full too (the particle is lost and the call reports it).
 template <typename T>
 __device__ __forceinline__ bool put_particle(const BinnedArgs<T> &a,
-                                             const BinnedSpecies<T> &sp, int tile,
-                                             unsigned int stay_base, bool stay,
-                                             unsigned int j, int dest, T x, T y, T z,
-                                             T u, T v, T w, T q, uint32_t pid) {
-  int64_t o = -1;
+                                             const BinnedSpecies<T> &sp, uint32_t own_lo,
+                                             uint32_t own_cap, unsigned int stay_base,
+                                             bool stay, unsigned int j, int dest, T x, T y,
+                                             T z, T u, T v, T w, T q, uint32_t pid) {
+  uint32_t o;
   if (stay) {
-    const int64_t k = (int64_t)stay_base + j;
-    if (k < gmem(sp.seg)[tile + 1] - gmem(sp.seg)[tile]) o = gmem(sp.seg)[tile] + k;
+    const unsigned int k = stay_base + j;
+    if (k < own_cap) {
+      o = own_lo + k;
+    } else {
+      const unsigned int e = gatomic_add(sp.cnt_out + a.ntiles, 1u);
+      o = e < sp.ovf_cap ? (uint32_t)gmem(sp.seg)[a.ntiles] + e : kNoSlot;
+    }
   } else {
-    const unsigned int k = gatomic_add(sp.cnt_out + dest, 1u);
-    if ((int64_t)k < gmem(sp.seg)[dest + 1] - gmem(sp.seg)[dest]) o = gmem(sp.seg)[dest] + k;
+    o = seg_slot(a, sp, dest, gatomic_add(sp.cnt_out + dest, 1u));
   }
-  if (o < 0) {
-    const unsigned int k = gatomic_add(sp.cnt_out + a.ntiles, 1u);
-    if (k < sp.ovf_cap) o = gmem(sp.seg)[a.ntiles] + (int64_t)k;
-  }
-  if (o < 0) return false;
+  if (o == kNoSlot) return false;
   write_particle(sp, o, x, y, z, u, v, w, q, pid);
   return true;
 }
@@ -699,6 +768,10 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
   blk.fld = s_fld;
   blk.xn = s_xn; blk.yn = s_yn; blk.zn = s_zn;
   blk.c0x = tx * TX; blk.c0y = ty * TY; blk.c0z = tz * TZ;
+  // last lower-node index (block coordinates) of a stencil inside both the
+  // staged block and the grid: a stencil within these needs no clamp
+  const int lim_b[3] = {min(BX - 2, a.g.nxn - 2 - blk.c0x), min(BY - 2, a.g.nyn - 2 - blk.c0y),
+                        min(BZ - 2, a.g.nzn - 2 - blk.c0z)};
   blk.gptr = (const SPIC_GLOBAL T *const volatile __attribute__((address_space(3))) *)s_gptr;
   if (tid == 0) {
     s_gptr[0] = gmem(a.f.Ex); s_gptr[1] = gmem(a.f.Ey); s_gptr[2] = gmem(a.f.Ez);
@@ -740,8 +813,8 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
 
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const RangeView in = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, tile);
-    const int64_t nin = in.count;
+    const SlotRange in = slot_range(a, sp, tile);
+    const uint32_t nin = in.count;
     if (tid == 0) gmem(a.pflag)[(size_t)tile * a.ns + s] = nin > 0 ? 1 : 0;
     if (nin == 0) continue;  // uniform across the workgroup
     for (int i = tid; i < 10 * NB; i += NT) s_mom[i] = A(0);
@@ -758,33 +831,36 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     unsigned int lst_done = 0u;
     __syncthreads();
     const unsigned int stay_base = s_base;
+    // this tile's output segment
+    const uint32_t own_lo = (uint32_t)gmem(sp.seg)[tile];
+    const uint32_t own_cap = (uint32_t)gmem(sp.seg)[tile + 1] - own_lo;
     T acc[20];  // register sums, particle precision
 #pragma unroll
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
-    const int64_t nbatches = (nin + NT - 1) / NT;
+    const uint32_t nbatches = (nin + NT - 1) / NT;
     // this thread's particle of batch j (x y z u v w q and its id); with
     // PREFETCH (FP32, which has the registers for it at three waves per
     // SIMD; FP64 would spill) batch j + 1 is loaded while batch j is pushed
     constexpr bool PREFETCH = sizeof(T) == 4;
     T nxt[7];
     uint32_t nxt_id = 0;
-    auto load_particle = [&](int64_t r) {
+    auto load_particle = [&](uint32_t r) {
 #pragma unroll
       for (int c = 0; c < 7; ++c) nxt[c] = T(0);
       nxt_id = 0;
       if (r < nin) {
-        const int64_t i = in.slot(r);
-        const SPIC_GLOBAL T *p = gmem(sp.in) + i;
-        const int64_t st = sp.stride;
+        const uint32_t i = in.slot(r);
+        const uint32_t off = i * (uint32_t)sizeof(T);
 #pragma unroll
-        for (int c = 0; c < 7; ++c) nxt[c] = p[c * st];
-        nxt_id = gmem(sp.id_in)[i];
+        for (int c = 0; c < 7; ++c) nxt[c] = load_comp(sp, c, off);
+        nxt_id = *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) +
+                                                 i * 4u);
       }
     };
     if (PREFETCH) load_particle(tid);
-    for (int64_t j = 0; j < nbatches; ++j) {
-      const int64_t r = j * NT + tid;
+    for (uint32_t j = 0; j < nbatches; ++j) {
+      const uint32_t r = j * NT + tid;
       const bool active = r < nin;
       const int buf = (int)(j & 1);
       if (!PREFETCH) load_particle(r);
@@ -796,7 +872,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
         if constexpr (FX) {
           int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
           if constexpr (PUSH)
-            push_fixed<true, TX, TY, TZ>(a.g, a.f, sp.k, a.fx, blk, X, Y, Z, u, v, w,
+            push_fixed<true, TX, TY, TZ>(a.g, a.f, sp.k, a.fx, blk, lim_b, X, Y, Z, u, v, w,
                                          bad_move, n_lg, n_gg);
           ix = fx_deposit_cell(X, a.fx.k[0], a.g.nxn, bad_dep);
           iy = fx_deposit_cell(Y, a.fx.k[1], a.g.nyn, bad_dep);
@@ -877,8 +953,8 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           else ++n_gd;
           const bool stay = dest == tile;
           const unsigned int jj = stay ? atomicAdd(&s_stay, 1u) : 0u;
-          bad_store |= !put_particle(a, sp, tile, stay_base, stay, jj, dest, x, y, z, u, v,
-                                     w, q, pid);
+          bad_store |= !put_particle(a, sp, own_lo, own_cap, stay_base, stay, jj, dest, x, y,
+                                     z, u, v, w, q, pid);
         }
       }
 
@@ -893,8 +969,8 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           jb = __shfl(jb, leader);
           if (bucketed) {
             const unsigned int jj = jb + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
-            bad_store |= !put_particle(a, sp, tile, stay_base, true, jj, tile, x, y, z, u,
-                                       v, w, q, pid);
+            bad_store |= !put_particle(a, sp, own_lo, own_cap, stay_base, true, jj, tile, x,
+                                       y, z, u, v, w, q, pid);
           }
         }
       }
@@ -972,8 +1048,8 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
               const int pdest = tile_of<TX, TY, TZ>(a, jx, jy, jz);
               const bool stay = pdest == tile;
               const unsigned int jj = stay ? atomicAdd(&s_stay, 1u) : 0u;
-              bad_store |= !put_particle(a, sp, tile, stay_base, stay, jj, pdest, px, py,
-                                         pz, pu, pv, pw, pq, s_lid[e]);
+              bad_store |= !put_particle(a, sp, own_lo, own_cap, stay_base, stay, jj, pdest,
+                                         px, py, pz, pu, pv, pw, pq, s_lid[e]);
             }
           }
           lst_done = nl;
@@ -1061,6 +1137,7 @@ template <typename T, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int part,
                                                      int nparts) {
   Block<T, 1, 1, 1> none;
+  const int no_lim[3] = {-1, -1, -1};  // nothing is staged
   none.fld = nullptr;
   none.xn = none.yn = none.zn = nullptr;
   none.c0x = none.c0y = none.c0z = 0;
@@ -1068,23 +1145,22 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
   unsigned int n_gg = 0, n_gd = 0, n_lg = 0;
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
-    const RangeView ov = range_view(gmem(sp.seg), gmem(sp.cnt_in), a.ntiles, sp.ovf_cap, a.ntiles);
-    const int64_t nin = ov.count;
-    for (int64_t r = part * (int64_t)256 + threadIdx.x; r < nin;
-         r += (int64_t)nparts * 256) {
-      const int64_t i = ov.slot(r);
-      const SPIC_GLOBAL T *p = gmem(sp.in) + i;
-      const int64_t st = sp.stride;
-      T x = p[0], y = p[st], z = p[2 * st];
-      T u = p[3 * st], v = p[4 * st], w = p[5 * st];
-      const T q = p[6 * st];
-      const uint32_t pid = gmem(sp.id_in)[i];
+    const SlotRange ov = slot_range(a, sp, a.ntiles);
+    const uint32_t nin = ov.count;
+    for (uint32_t r = part * 256u + threadIdx.x; r < nin; r += (uint32_t)nparts * 256u) {
+      const uint32_t i = ov.slot(r);
+      const uint32_t off = i * (uint32_t)sizeof(T);
+      T x = load_comp(sp, 0, off), y = load_comp(sp, 1, off), z = load_comp(sp, 2, off);
+      T u = load_comp(sp, 3, off), v = load_comp(sp, 4, off), w = load_comp(sp, 5, off);
+      const T q = load_comp(sp, 6, off);
+      const uint32_t pid =
+          *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) + i * 4u);
       int ix, iy, iz;
       if constexpr (FX) {
         int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
         if constexpr (PUSH)
-          push_fixed<false, 1, 1, 1>(a.g, a.f, sp.k, a.fx, none, X, Y, Z, u, v, w, bad_move,
-                                     n_lg, n_gg);
+          push_fixed<false, 1, 1, 1>(a.g, a.f, sp.k, a.fx, none, no_lim, X, Y, Z, u, v, w,
+                                     bad_move, n_lg, n_gg);
         ix = fx_deposit_cell(X, a.fx.k[0], a.g.nxn, bad_dep);
         iy = fx_deposit_cell(Y, a.fx.k[1], a.g.nyn, bad_dep);
         iz = fx_deposit_cell(Z, a.fx.k[2], a.g.nzn, bad_dep);
@@ -1103,8 +1179,8 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
         bad_dep |= bclamp(iz, a.g.nzn);
       }
       const int dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
-      const int64_t o = out_slot(a, sp, dest, gatomic_add(sp.cnt_out + dest, 1u));
-      if (o >= 0) write_particle(sp, o, x, y, z, u, v, w, q, pid);
+      const uint32_t o = seg_slot(a, sp, dest, gatomic_add(sp.cnt_out + dest, 1u));
+      if (o != kNoSlot) write_particle(sp, o, x, y, z, u, v, w, q, pid);
       else bad_store = true;
       ++n_gd;
       if constexpr (FX)

@@ -472,6 +472,13 @@ int spic_engine_load_as(spic_engine *e, int is, const spic_species *src, int pre
   if (slack < 16) slack = 16;
   // segments (n + nt * slack) then the overflow region (n)
   const int64_t cap = 2 * n + nt * slack;
+  // the phase kernels address a slot by its 32-bit byte offset within a
+  // component array
+  if ((uint64_t)cap * (uint64_t)e->esz > 0xffffffffull)
+    return fail(SPIC_EUNSUPPORTED,
+                "species of %lld particles: its %lld slots per component exceed 4 GiB "
+                "(at most about %lld particles per species at this precision)",
+                (long long)n, (long long)cap, (long long)(0xffffffffull / e->esz / 2.2));
   hipError_t err;
   // storage is kept while it holds `cap` slots (a smaller species loaded
   // after a larger one reuses it with the smaller component stride)
