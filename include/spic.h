@@ -48,9 +48,9 @@ enum {
   SPIC_EHIP = 2,        /* HIP runtime error */
   SPIC_EDOMAIN = 3,     /* a particle left the grid by more than one cell:
                            the reference would index outside its arrays */
-  SPIC_EUNSUPPORTED = 4,/* a configuration the reference does not run the same
-                           way (n_sub_cycles != 1: the CPU mover has no
-                           sub-cycle loop, src/Mover.cpp:11) */
+  SPIC_EUNSUPPORTED = 4,/* a configuration outside what is built (e.g.
+                           n_sub_cycles != 1 while sub-cycling is refused,
+                           see spic_set_subcycling) */
   SPIC_ENOMEM = 5       /* device allocation failed */
 };
 
@@ -467,6 +467,25 @@ int spic_set_async(int async_mode);
  * and operand types, so the particle state matches the CPU reference bit
  * for bit. */
 int spic_set_strict_fp(int strict);
+/* Sub-cycling of the mover, for species with n_sub_cycles != 1.  The
+ * reference's two movers differ here: the CPU mover has no sub-cycle loop
+ * and pushes once with dt / n_sub_cycles (src/Mover.cpp:11-12); the GPU
+ * mover pushes n_sub_cycles times with that step, applying the particle BC
+ * after each push (src/gpu/Particles_gpu.cu:319-437).
+ *   SPIC_SUBCYCLE_REFUSE (default): n_sub_cycles != 1 is refused with
+ *     SPIC_EUNSUPPORTED;
+ *   SPIC_SUBCYCLE_LOOP: the GPU mover's loop (step (FPpart)dt /
+ *     n_sub_cycles, in the particle precision as that mover forms it);
+ *   SPIC_SUBCYCLE_SINGLE: the CPU mover's single push of dt / n_sub_cycles.
+ * Both BC tests of an axis are applied after each push, as the CPU mover
+ * does (the GPU mover's `else if` gives the same result for any
+ * displacement shorter than the box).  Read whenever push constants are
+ * formed: by each mover call and engine phase, and by a batch when it is
+ * created or its tables are rebuilt.  n_sub_cycles < 1 is always refused. */
+#define SPIC_SUBCYCLE_REFUSE 0
+#define SPIC_SUBCYCLE_LOOP 1
+#define SPIC_SUBCYCLE_SINGLE 2
+int spic_set_subcycling(int mode);
 /* In asynchronous mode: wait for `stream`, return SPIC_EDOMAIN if any
  * out-of-domain particle was seen since the last check, and clear it. */
 int spic_check_domain(void *stream);

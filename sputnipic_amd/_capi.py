@@ -25,6 +25,9 @@ SPIC_SORT_TILE = 0
 SPIC_SORT_RADIX = 1
 SPIC_POS_FLOAT = 1
 SPIC_POS_FIXED = 2
+SPIC_SUBCYCLE_REFUSE = 0
+SPIC_SUBCYCLE_LOOP = 1
+SPIC_SUBCYCLE_SINGLE = 2
 
 STATUS_NAMES = {
     SPIC_OK: "SPIC_OK", SPIC_EINVAL: "SPIC_EINVAL", SPIC_EHIP: "SPIC_EHIP",
@@ -189,6 +192,7 @@ SIGNATURES = {
     "spic_abi_version": (C.c_int, []),
     "spic_set_async": (C.c_int, [C.c_int]),
     "spic_set_strict_fp": (C.c_int, [C.c_int]),
+    "spic_set_subcycling": (C.c_int, [C.c_int]),
     "spic_check_domain": (C.c_int, [_P]),
     "spic_test_fail_engine_alloc": (C.c_int, [C.c_int]),
     "spic_device_count": (C.c_int, [C.POINTER(C.c_int)]),

@@ -349,6 +349,15 @@ def set_strict_fp(strict: bool) -> None:
     check(lib().spic_set_strict_fp(int(bool(strict))), "spic_set_strict_fp")
 
 
+def set_subcycling(mode: int) -> None:
+    """Sub-cycling of the mover for n_sub_cycles != 1 (spic_set_subcycling):
+    SPIC_SUBCYCLE_REFUSE (default), SPIC_SUBCYCLE_LOOP (the reference GPU
+    mover's loop, src/gpu/Particles_gpu.cu:336-437) or SPIC_SUBCYCLE_SINGLE
+    (the reference CPU mover's single step of dt / n_sub_cycles,
+    src/Mover.cpp:11)."""
+    check(lib().spic_set_subcycling(int(mode)), "spic_set_subcycling")
+
+
 _ASYNC = [False]
 
 

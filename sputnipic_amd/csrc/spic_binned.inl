@@ -138,7 +138,7 @@ struct Block {
 // boundary conditions (src/Mover.cpp:33-157), in place.  With STAGED the
 // stencil comes from the block when it lies inside it.
 template <typename T, bool STAGED, int TX, int TY, int TZ>
-__device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T> &f,
+__device__ __forceinline__ void push_particle_once(const KGrid<T> &g, const KField<T> &f,
                                               const KPush<T> &k,
                                               const Block<T, TX, TY, TZ> &blk, T &x,
                                               T &y, T &z, T &u, T &v, T &w,
@@ -285,6 +285,20 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
   }
 }
 
+// The push once, or k.nloop times with the particle BC after each (the
+// reference GPU mover's sub-cycle loop, src/gpu/Particles_gpu.cu:336-437).
+template <typename T, bool STAGED, int TX, int TY, int TZ>
+__device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T> &f,
+                                              const KPush<T> &k,
+                                              const Block<T, TX, TY, TZ> &blk, T &x, T &y,
+                                              T &z, T &u, T &v, T &w, bool &bad,
+                                              unsigned int &n_lds, unsigned int &n_glob) {
+#pragma unroll 1
+  for (int isub = 0; isub < k.nloop; ++isub)
+    push_particle_once<T, STAGED, TX, TY, TZ>(g, f, k, blk, x, y, z, u, v, w, bad, n_lds,
+                                              n_glob);
+}
+
 // push_particle for fixed-point positions (FP32 form): the same
 // predictor-corrector iteration, rotation and final update, with the
 // interpolation weights as offsets within the cell in cell units (so that
@@ -303,7 +317,7 @@ __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T>
 // gathers from global memory.  The cached stencil is keyed by its lower
 // block node (staged) or by its packed global cell (not staged).
 template <bool STAGED, int TX, int TY, int TZ>
-__device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<float> &f,
+__device__ __forceinline__ void push_fixed_once(const KGrid<float> &g, const KField<float> &f,
                                            const KPush<float> &k, const FixedParams &fp,
                                            const Block<float, TX, TY, TZ> &blk,
                                            const int lim_b[3], int &X, int &Y, int &Z,
@@ -462,6 +476,19 @@ __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<f
     if (k.perz) Z += fp.lim[2];
     else { w = -w; Z = -Z; }
   }
+}
+
+template <bool STAGED, int TX, int TY, int TZ>
+__device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<float> &f,
+                                           const KPush<float> &k, const FixedParams &fp,
+                                           const Block<float, TX, TY, TZ> &blk,
+                                           const int lim_b[3], int &X, int &Y, int &Z,
+                                           float &u, float &v, float &w, bool &bad,
+                                           unsigned int &n_lds, unsigned int &n_glob) {
+#pragma unroll 1
+  for (int isub = 0; isub < k.nloop; ++isub)
+    push_fixed_once<STAGED, TX, TY, TZ>(g, f, k, fp, blk, lim_b, X, Y, Z, u, v, w, bad, n_lds,
+                                        n_glob);
 }
 // Tile of the deposit cell (ix, iy, iz), clamped to the physical cells;
 // the tile shape is a compile-time constant (divisions become shifts).
@@ -868,15 +895,27 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       uint32_t pid = nxt_id;
       if (PREFETCH && j + 1 < nbatches) load_particle(r + NT);
       int ix = 2, iy = 2, iz = 2, dest = -1;
+      bool in_tile = false;  // the deposit cell is one of the tile's cells
       if (active) {
         if constexpr (FX) {
           int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
           if constexpr (PUSH)
             push_fixed<true, TX, TY, TZ>(a.g, a.f, sp.k, a.fx, blk, lim_b, X, Y, Z, u, v, w,
                                          bad_move, n_lg, n_gg);
-          ix = fx_deposit_cell(X, a.fx.k[0], a.g.nxn, bad_dep);
-          iy = fx_deposit_cell(Y, a.fx.k[1], a.g.nyn, bad_dep);
-          iz = fx_deposit_cell(Z, a.fx.k[2], a.g.nzn, bad_dep);
+          // deposit cell 2 + floor (src/Mover.cpp:411-413) by a shift; a
+          // cell of the tile is valid as it is, any other is clamped and
+          // flagged as fx_deposit_cell does
+          ix = 2 + (X >> a.fx.k[0]);
+          iy = 2 + (Y >> a.fx.k[1]);
+          iz = 2 + (Z >> a.fx.k[2]);
+          in_tile = ((unsigned)(ix - 2 - blk.c0x) < (unsigned)TX) &
+                    ((unsigned)(iy - 2 - blk.c0y) < (unsigned)TY) &
+                    ((unsigned)(iz - 2 - blk.c0z) < (unsigned)TZ);
+          if (!in_tile) {
+            bad_dep |= bclamp(ix, a.g.nxn);
+            bad_dep |= bclamp(iy, a.g.nyn);
+            bad_dep |= bclamp(iz, a.g.nzn);
+          }
           x = __int_as_float(X);
           y = __int_as_float(Y);
           z = __int_as_float(Z);
@@ -890,27 +929,29 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           bad_dep |= bclamp(ix, a.g.nxn);
           bad_dep |= bclamp(iy, a.g.nyn);
           bad_dep |= bclamp(iz, a.g.nzn);
+          in_tile = ((unsigned)(ix - 2 - blk.c0x) < (unsigned)TX) &
+                    ((unsigned)(iy - 2 - blk.c0y) < (unsigned)TY) &
+                    ((unsigned)(iz - 2 - blk.c0z) < (unsigned)TZ);
         }
-        dest = tile_of<TX, TY, TZ>(a, ix, iy, iz);
+        dest = in_tile ? tile : tile_of<TX, TY, TZ>(a, ix, iy, iz);
       }
 
       // deposit: bucket (tile cell), else the list ring (halo cell,
       // outside the block, full bucket), else directly
       bool bucketed = false;
       if (active) {
-        const int lcx = ix - 2 - blk.c0x, lcy = iy - 2 - blk.c0y, lcz = iz - 2 - blk.c0z;
-        const bool in_tile = (lcx >= 0) & (lcx < TX) & (lcy >= 0) & (lcy < TY) &
-                             (lcz >= 0) & (lcz < TZ);
         bool done = false;
         if (in_tile) {
-          const int cell = (lcx * TY + lcy) * TZ + lcz;
+          const int cell = ((ix - 2 - blk.c0x) * TY + (iy - 2 - blk.c0y)) * TZ + (iz - 2 - blk.c0z);
           const unsigned int sl = atomicAdd(&s_cnt[buf][cell], 1u);
           if (sl < (unsigned int)CAP) {
             const int e = cell * CS + (int)sl;
             if constexpr (FX) {
-              s_bkt[0][e] = fx_frac(__float_as_int(x), ix - 2, a.fx.k[0], a.fx.unit[0]);
-              s_bkt[1][e] = fx_frac(__float_as_int(y), iy - 2, a.fx.k[1], a.fx.unit[1]);
-              s_bkt[2][e] = fx_frac(__float_as_int(z), iz - 2, a.fx.k[2], a.fx.unit[2]);
+              // offsets within the (unclamped) cell: the fixed-point
+              // coordinate's low bits
+              s_bkt[0][e] = (float)(__float_as_int(x) & ((1 << a.fx.k[0]) - 1)) * a.fx.unit[0];
+              s_bkt[1][e] = (float)(__float_as_int(y) & ((1 << a.fx.k[1]) - 1)) * a.fx.unit[1];
+              s_bkt[2][e] = (float)(__float_as_int(z) & ((1 << a.fx.k[2]) - 1)) * a.fx.unit[2];
             } else {
               s_bkt[0][e] = x; s_bkt[1][e] = y; s_bkt[2][e] = z;
             }
@@ -984,14 +1025,20 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
         const T Y0 = s_yn[mly + 1], Y1 = s_yn[mly + 2];
         const T Z0 = s_zn[mlz + 1], Z1 = s_zn[mlz + 2];
         const T iv = a.g.invVOL;
+        // xa = px - X0 (a = 0) or X1 - px (a = 1), as one fused multiply-add
+        // with this thread's constants (exact: a product by +-1 then one
+        // rounding); with FX px is already the offset: px or 1 - px
+        const T sa = my_a ? T(-1) : T(1), sb = my_b ? T(-1) : T(1);
+        const T ca = FX ? (my_a ? T(1) : T(0)) : (my_a ? X1 : -X0);
+        const T cb = FX ? (my_b ? T(1) : T(0)) : (my_b ? Y1 : -Y0);
         for (unsigned int k = 0; k < nb_c; ++k) {
           const int e = my_cell * CS + (int)k;
           const T px = s_bkt[0][e], py = s_bkt[1][e], pz = s_bkt[2][e];
           const T pq = s_bkt[3][e], pu = s_bkt[4][e], pv = s_bkt[5][e], pw = s_bkt[6][e];
           // px py pz: the particle's position, or with FX its offsets
           // within the cell (the distances from nodes X0 Y0 Z0 in cells)
-          const T xa = FX ? (my_a == 0 ? px : T(1) - px) : (my_a == 0 ? px - X0 : X1 - px);
-          const T yb = FX ? (my_b == 0 ? py : T(1) - py) : (my_b == 0 ? py - Y0 : Y1 - py);
+          const T xa = fma(sa, px, ca);
+          const T yb = fma(sb, py, cb);
           const T uu = pu * pu, uv = pu * pv, uw = pu * pw, vv = pv * pv, vw = pv * pw,
                   ww = pw * pw;
           if (kStrictB) {
@@ -1005,7 +1052,8 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
             // offsets are in cells, iv) is applied once when the sums are
             // added to the LDS block
             const T qxy = pq * xa * yb;
-            const T w0 = qxy * (FX ? pz : pz - Z0), w1 = qxy * (FX ? T(1) - pz : Z1 - pz);
+            const T w0 = qxy * (FX ? pz : pz - Z0);
+            const T w1 = FX ? qxy - w0 : qxy * (Z1 - pz);
             const T pm[10] = {T(1), pu, pv, pw, uu, uv, uw, vv, vw, ww};
 #pragma unroll
             for (int m = 0; m < 10; ++m) {

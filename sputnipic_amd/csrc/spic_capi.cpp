@@ -41,6 +41,7 @@ namespace {
 
 std::atomic<int> g_async{0};
 std::atomic<int> g_strict{0};
+std::atomic<int> g_subcycle{SPIC_SUBCYCLE_REFUSE};
 using spic::fail;
 using spic::hip_fail;
 using spic::kgrid;
@@ -149,10 +150,12 @@ int check_species(const spic_species *sp, bool need_q, bool need_move) {
   if (need_move) {
     if (sp->NiterMover < 1)
       return fail(SPIC_EINVAL, "NiterMover must be >= 1");
-    if (sp->n_sub_cycles != 1)
+    if (sp->n_sub_cycles < 1) return fail(SPIC_EINVAL, "n_sub_cycles must be >= 1");
+    if (sp->n_sub_cycles != 1 && g_subcycle.load() == SPIC_SUBCYCLE_REFUSE)
       return fail(SPIC_EUNSUPPORTED,
-                  "n_sub_cycles = %d: the reference CPU mover has no sub-cycle "
-                  "loop (src/Mover.cpp:11), only n_sub_cycles = 1 is defined",
+                  "n_sub_cycles = %d: the reference's CPU mover pushes once with dt / "
+                  "n_sub_cycles (src/Mover.cpp:11), its GPU mover loops; choose one with "
+                  "spic_set_subcycling",
                   sp->n_sub_cycles);
   }
   return SPIC_OK;
@@ -190,6 +193,7 @@ using spic::check_push;
 namespace spic {
 bool async_mode() { return g_async.load() != 0; }
 bool strict_mode() { return g_strict.load() != 0; }
+int subcycle_mode() { return g_subcycle.load(); }
 
 // Prepare the status words for one call; returns the device pointer.
 int begin_call(hipStream_t s, unsigned int **st) {
@@ -251,6 +255,14 @@ int spic_set_async(int async_mode) {
 
 int spic_set_strict_fp(int strict) {
   g_strict.store(strict ? 1 : 0);
+  return SPIC_OK;
+}
+
+int spic_set_subcycling(int mode) {
+  if (mode != SPIC_SUBCYCLE_REFUSE && mode != SPIC_SUBCYCLE_LOOP &&
+      mode != SPIC_SUBCYCLE_SINGLE)
+    return fail(SPIC_EINVAL, "sub-cycling mode must be SPIC_SUBCYCLE_REFUSE, _LOOP or _SINGLE");
+  g_subcycle.store(mode);
   return SPIC_OK;
 }
 

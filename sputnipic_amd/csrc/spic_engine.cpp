@@ -313,6 +313,18 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   return hipSuccess;
 }
 
+// A species loaded with n_sub_cycles != 1 runs only while a sub-cycling
+// mode is chosen (the mode may have changed since the load).
+int check_subcycles(const spic_engine *e) {
+  if (spic::subcycle_mode() != SPIC_SUBCYCLE_REFUSE) return SPIC_OK;
+  for (int is = 0; is < e->ns; ++is)
+    if (e->sp[is].loaded && e->sp[is].nsub != 1)
+      return fail(SPIC_EUNSUPPORTED,
+                  "species %d has n_sub_cycles = %d and sub-cycling is refused "
+                  "(spic_set_subcycling)", is, e->sp[is].nsub);
+  return SPIC_OK;
+}
+
 // Strict arithmetic reproduces the reference's rounding of lengths, which
 // fixed-point storage does not hold.
 int check_positions(const spic_engine *e) {
@@ -617,7 +629,7 @@ int spic_engine_phase(spic_engine *e, const spic_field *field,
     if (!e->sp[is].loaded) continue;
     if ((rc = spic::check_moments(&mom[is]))) return rc;
   }
-  if ((rc = check_positions(e))) return rc;
+  if ((rc = check_positions(e)) || (rc = check_subcycles(e))) return rc;
   std::lock_guard<std::mutex> lock(e->mu);
   hipStream_t s = (hipStream_t)stream;
   hipError_t err;
@@ -908,6 +920,7 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
         return fail(SPIC_EINVAL, "batch members %d and %d share an engine", j, i);
     for (int is = 0; is < e->ns; ++is)
       if (e->sp[is].loaded && (rc = spic::check_moments(&m.moments[is]))) return rc;
+    if ((rc = check_subcycles(e))) return rc;
   }
   spic_batch *b = new (std::nothrow) spic_batch();
   if (!b) return fail(SPIC_ENOMEM, "batch allocation");

@@ -44,6 +44,7 @@ struct KPush {
   T dto2;    // .5 * dt_sub
   T qomdt2;  // qom * dto2 / c
   int niter;
+  int nloop;  // pushes per call: n_sub_cycles in SPIC_SUBCYCLE_LOOP, else 1
   int perx, pery, perz;
 };
 
@@ -121,14 +122,24 @@ inline KMom kmom(const spic_moments *m) {
   return k;
 }
 
+int subcycle_mode();
+
 // Push constants in the reference's types (src/Mover.cpp:11-12):
 //   FPpart dt_sub = (FPpart)dt / ((double)n_sub_cycles);
 //   FPpart dto2 = .5 * dt_sub, qomdt2 = qom * dto2 / c;   (qom is FPpart)
+// With the GPU mover's sub-cycle loop the step is formed as that mover forms
+// it, (FPpart)dt / ((float)n_sub_cycles) (src/gpu/Particles_gpu.cu:319), and
+// the push runs n_sub_cycles times.
 template <typename T>
 inline KPush<T> kpush(const spic_species *sp, const spic_push *p) {
   KPush<T> k;
   const T qom = (T)sp->qom;
-  k.dt_sub = (T)((double)(T)p->dt / ((double)sp->n_sub_cycles));
+  const bool loop = subcycle_mode() == SPIC_SUBCYCLE_LOOP;
+  k.nloop = loop ? sp->n_sub_cycles : 1;
+  if (loop)
+    k.dt_sub = (T)((T)p->dt / (T)(float)sp->n_sub_cycles);
+  else
+    k.dt_sub = (T)((double)(T)p->dt / ((double)sp->n_sub_cycles));
   k.dto2 = (T)(.5 * (double)k.dt_sub);
   const T qd = qom * k.dto2;  // FPpart * FPpart
   k.qomdt2 = (T)((double)qd / p->c);

@@ -64,7 +64,7 @@ __device__ __forceinline__ bool clamp_index(int &ix, int n) {
 // (src/Mover.cpp:33-157).  Returns true if an iterate left the grid by
 // more than one cell.
 template <typename T>
-__device__ __forceinline__ bool push_particle(const KGrid<T> &g,
+__device__ __forceinline__ bool push_particle_once(const KGrid<T> &g,
                                               const KField<T> &f,
                                               const KPush<T> &k, T &x, T &y,
                                               T &z, T &u, T &v, T &w) {
@@ -158,6 +158,20 @@ __device__ __forceinline__ bool push_particle(const KGrid<T> &g,
     if (k.perz) z = (T)((double)z + g.Lz);
     else { w = -w; z = -z; }
   }
+  return bad;
+}
+
+// The push of one particle: once, or k.nloop times with the particle BC
+// after each (the reference GPU mover's sub-cycle loop,
+// src/gpu/Particles_gpu.cu:336-437).
+template <typename T>
+__device__ __forceinline__ bool push_particle(const KGrid<T> &g, const KField<T> &f,
+                                              const KPush<T> &k, T &x, T &y, T &z, T &u,
+                                              T &v, T &w) {
+  bool bad = false;
+#pragma unroll 1
+  for (int isub = 0; isub < k.nloop; ++isub)
+    bad |= push_particle_once<T>(g, f, k, x, y, z, u, v, w);
   return bad;
 }
 

@@ -96,13 +96,37 @@ def test_null_arguments_are_rejected():
 
 def test_sub_cycling_is_refused():
     """The CPU mover has no sub-cycle loop (src/Mover.cpp:11) while the GPU
-    reference has one (src/gpu/Particles_gpu.cu:336): n_sub_cycles != 1 is
-    refused rather than silently following either."""
+    reference has one (src/gpu/Particles_gpu.cu:336): by default
+    n_sub_cycles != 1 is refused rather than silently following either."""
     L = _capi.lib()
     g, f, s, p = _fake_args(n_sub_cycles=2)
     assert L.spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), C.byref(p),
                            None) == _capi.SPIC_EUNSUPPORTED
     assert b"n_sub_cycles" in L.spic_last_error()
+
+
+def test_sub_cycling_modes_pass_the_species_check():
+    """With a sub-cycling mode chosen, a species with n_sub_cycles = 2 passes
+    the species check: the call stops at the next check (a NULL push
+    descriptor, so nothing reaches the device); n_sub_cycles = 0 and an
+    unknown mode are refused."""
+    L = _capi.lib()
+    g, f, s, p = _fake_args(n_sub_cycles=2)
+    try:
+        assert L.spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), None,
+                               None) == _capi.SPIC_EUNSUPPORTED
+        for mode in (_capi.SPIC_SUBCYCLE_LOOP, _capi.SPIC_SUBCYCLE_SINGLE):
+            assert L.spic_set_subcycling(mode) == _capi.SPIC_OK
+            assert L.spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), None,
+                                   None) == _capi.SPIC_EINVAL
+            assert b"n_sub_cycles" not in L.spic_last_error()
+        g, f, s, p = _fake_args(n_sub_cycles=0)
+        assert L.spic_mover_pc(C.byref(g), C.byref(f), C.byref(s), None,
+                               None) == _capi.SPIC_EINVAL
+        assert b"n_sub_cycles" in L.spic_last_error()
+        assert L.spic_set_subcycling(7) == _capi.SPIC_EINVAL
+    finally:
+        L.spic_set_subcycling(_capi.SPIC_SUBCYCLE_REFUSE)
 
 
 @pytest.mark.parametrize("field,value", [("prec", 7), ("nxn", 10), ("Lx", -1.0)])
