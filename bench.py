@@ -59,7 +59,7 @@ def measured_traffic(prec: str):
 
 
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
-             tile) -> dict:
+             tile, order: str = "load") -> dict:
     import ctypes as C
 
     import sputnipic_amd as S
@@ -86,7 +86,10 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
     eng = None
     if engine == "tiled":
         # the phase writes the moments, so a cycle needs no zeroing pass
-        eng = S.Engine(grd, NS, tile=tile, sort_every=0, set_moments=True)
+        # order "engine": no per-particle load index (particles stay in the
+        # engine's tile order, as the reference's arrays after particle_sort)
+        eng = S.Engine(grd, NS, tile=tile, sort_every=0, set_moments=True,
+                       order=S.SPIC_ORDER_ENGINE if order == "engine" else S.SPIC_ORDER_LOAD)
         t0 = time.perf_counter()
         for s, p in enumerate(parts):
             eng.load(s, p, stream)
@@ -97,7 +100,8 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
                 d.free()
         parts = None
         kernel = (f"k_binned_phase<{prec}, {tile[0]}x{tile[1]}x{tile[2]}> (all species, one "
-                  f"launch; timed with its overflow, reduce and count-reset launches)")
+                  f"launch; timed with its overflow, reduce and count-reset launches; "
+                  f"particle order: {order})")
     else:
         s_c = [p.c_struct() for p in parts]
         kernel = "k_mover_deposit (plain fused, one launch per species)"
@@ -930,6 +934,11 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
+    ap.add_argument("--order", choices=("load", "engine"), default="load",
+                    help="engine particle order of the timed forms: load (a load index per "
+                         "particle, store in load order; default) or engine (no index)")
+    ap.add_argument("--no-engine-order", action="store_true",
+                    help="skip the extra timing of both forms without the load index")
     ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=0,
                     help="stress: re-sorts by tile sort (0) or radix sort (1)")
     ap.add_argument("--workload", choices=("gem", "stress", "batch", "field"), default="gem",
@@ -1054,7 +1063,14 @@ def main() -> int:
     for prec in args.forms.split(","):
         tile = tuple(int(t) for t in args.tile.split("x"))
         forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, args.sort_every,
-                               tile)
+                               tile, args.order)
+    # the same forms without the per-particle load index (reported beside the
+    # headline, which keeps the default engine)
+    forms_engine_order = {}
+    if args.engine == "tiled" and args.order == "load" and not args.no_engine_order:
+        for prec in args.forms.split(","):
+            forms_engine_order[prec] = run_form(prec, args.steps, args.warmup, args.engine,
+                                                args.sort_every, tile, "engine")
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
     cpu = refleg = None
@@ -1109,6 +1125,7 @@ def main() -> int:
         },
         "device_copy": copy,
         "forms": {k: {kk: vv for kk, vv in v.items()} for k, v in forms.items()},
+        "forms_engine_order": forms_engine_order or None,
         "cpu_baseline": cpu,
         "field_solve": field,
         "input_order": order,

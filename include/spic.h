@@ -264,6 +264,8 @@ typedef struct spic_engine spic_engine;
 #define SPIC_SORT_RADIX 1
 #define SPIC_POS_FLOAT 1
 #define SPIC_POS_FIXED 2
+#define SPIC_ORDER_LOAD 1
+#define SPIC_ORDER_ENGINE 2
 
 typedef struct spic_engine_opts {
   int tile_x, tile_y, tile_z; /* cells per tile: 4x4x4 (default, 0s) or 8x4x4 */
@@ -305,6 +307,24 @@ typedef struct spic_engine_opts {
                                  an FP32 engine created while strict
                                  arithmetic is off, else float.  Strict
                                  arithmetic refuses a fixed-point engine */
+  int order;                  /* SPIC_ORDER_LOAD (0, default): each particle
+                                 carries its load index (4 bytes read and
+                                 written per phase) so that store and the
+                                 snapshot return the load order;
+                                 SPIC_ORDER_ENGINE: no index, store and the
+                                 snapshot return the particles in the
+                                 engine's own order (tile by tile), as the
+                                 reference's particle_sort reorders its
+                                 arrays.  Trajectories and moments do not
+                                 depend on it */
+  int overflow_pct;           /* overflow slots of each buffer in percent of
+                                 the species' particles: 0 = default (100:
+                                 every particle fits, none can be lost);
+                                 1..99 bound the memory (at least 1024 slots)
+                                 and a phase whose leaving particles do not
+                                 fit their segments and the region reports
+                                 SPIC_ENOMEM (its output is then invalid:
+                                 load the species again) */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {

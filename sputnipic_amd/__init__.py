@@ -9,6 +9,7 @@ reference's interface over that ABI.  There is no CPU fallback.
 from ._capi import SpicError, SPIC_EDOMAIN, SPIC_EINVAL, SPIC_EUNSUPPORTED  # noqa: F401
 from ._capi import SPIC_POS_FIXED, SPIC_POS_FLOAT, SPIC_SORT_RADIX, SPIC_SORT_TILE  # noqa: F401
 from ._capi import SPIC_SUBCYCLE_LOOP, SPIC_SUBCYCLE_REFUSE, SPIC_SUBCYCLE_SINGLE  # noqa: F401
+from ._capi import SPIC_ORDER_ENGINE, SPIC_ORDER_LOAD  # noqa: F401
 from .core import (Batch, DeviceArray, DeviceParticles, EMField, Engine, Event, GemParams, Graph,  # noqa: F401
                    InterpDensAux, MaxwellImage, SolveParams, applyBCscalar, calculateHatDensities,
                    interpN2Crho_net,

@@ -25,6 +25,8 @@ SPIC_SORT_TILE = 0
 SPIC_SORT_RADIX = 1
 SPIC_POS_FLOAT = 1
 SPIC_POS_FIXED = 2
+SPIC_ORDER_LOAD = 1
+SPIC_ORDER_ENGINE = 2
 SPIC_SUBCYCLE_REFUSE = 0
 SPIC_SUBCYCLE_LOOP = 1
 SPIC_SUBCYCLE_SINGLE = 2
@@ -101,7 +103,7 @@ class spic_engine_opts(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int),
                 ("sort_every", C.c_int), ("lds_partials", C.c_int),
                 ("slack_pct", C.c_int), ("set_moments", C.c_int), ("sort_mode", C.c_int),
-                ("positions", C.c_int)]
+                ("positions", C.c_int), ("order", C.c_int), ("overflow_pct", C.c_int)]
 
 
 class spic_engine_stats(C.Structure):

@@ -110,8 +110,8 @@ struct BinnedSpecies {
   const T *in;
   T *out;
   int64_t stride;
-  const uint32_t *id_in;
-  uint32_t *id_out;
+  const uint32_t *id_in;  // load index per slot; both null when the engine
+  uint32_t *id_out;       // keeps no load order
   const int64_t *seg;          // [ntiles + 1]
   const unsigned int *cnt_in;  // [3 * (ntiles + 1)], see above
   unsigned int *cnt_out;       // [3 * (ntiles + 1)], zero on entry
@@ -169,15 +169,25 @@ struct CountArrays {
   int64_t nn;
 };
 
+// Work list of a batch phase's persistent launch: one 32-bit item per tile
+// of an instance or per part of its overflow regions, instance in the top
+// four bits.
+constexpr int kQueueInstShift = 28;
+constexpr uint32_t kQueueOverflowBit = 1u << 27;
+constexpr uint32_t kQueueIndexMask = kQueueOverflowBit - 1u;
+constexpr int kQueueOverflowParts = 64;  // overflow parts per instance
+static_assert(kMaxInstances <= (1 << (32 - kQueueInstShift)), "instance field too narrow");
+
 #define SPIC_DECLARE_BINNED_LAUNCHERS(NS)                                  \
   namespace NS {                                                           \
   template <typename T>                                                    \
   hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s);   \
   template <typename T>                                                    \
-  hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab,           \
-                                       const BatchMap &tiles,              \
-                                       const BatchMap &ovf, int tx, int ty,\
-                                       int tz, int lds_fp64, int fixed,    \
+  hipError_t launch_binned_phase_queue(const BinnedArgs<T> *tab,           \
+                                       const uint32_t *items,              \
+                                       uint32_t nitems, unsigned int *ctr, \
+                                       int tx, int ty, int tz,             \
+                                       int lds_fp64, int fixed,            \
                                        hipStream_t s);                     \
   }
 SPIC_DECLARE_BINNED_LAUNCHERS(spic_fast)
@@ -263,11 +273,21 @@ hipError_t bin_sort(const KGrid<T> &g, const BinLayout &L, int64_t n,
 
 // dst[id[i]] = src[i] for every particle in binned storage, seven arrays,
 // into FP64 arrays (dst_f64) or FP32 ones (FP32 engines only); fixed-point
-// positions are converted back to lengths.
+// positions are converted back to lengths.  Without ids (id == nullptr) the
+// particles are written densely in storage order, range t from base[t]
+// (range_bases).
 template <typename T>
 hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
-                     const int64_t *seg, const unsigned int *cnt,
+                     const uint32_t *base, const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, void *const dst[7], int dst_f64, hipStream_t s);
+
+// The first dense position of every range (tiles, then the overflow
+// region) of a binned species: an exclusive scan of its range counts into
+// the workspace (*bases points there).  With workspace == nullptr only
+// *needed is set.
+hipError_t range_bases(const int64_t *seg, const unsigned int *cnt, unsigned int ovf_cap,
+                       int ntiles, void *workspace, size_t workspace_bytes, size_t *needed,
+                       const uint32_t **bases, hipStream_t s);
 
 // Tile sort of one binned species (a job) from buffer src (layout
 // src_seg) to buffer dst (layout dst_seg): the binned source's alternative
@@ -323,7 +343,7 @@ hipError_t tile_sort_batch(const TileSortJob<T> *jobs, const TileSortMap &map, i
 // rounded to float): the particle datasets of a snapshot.
 template <typename T>
 hipError_t bin_store_be32(const BinLayout &L, T *const src[7], const uint32_t *id,
-                          const int64_t *seg, const unsigned int *cnt,
+                          const uint32_t *base, const int64_t *seg, const unsigned int *cnt,
                           unsigned int ovf_cap, uint32_t *const dst[6], hipStream_t s);
 
 }  // namespace spic

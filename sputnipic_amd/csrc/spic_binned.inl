@@ -557,7 +557,8 @@ __device__ __forceinline__ void write_particle(const BinnedSpecies<T> &sp, uint3
   store_comp(sp, 0, off, x); store_comp(sp, 1, off, y); store_comp(sp, 2, off, z);
   store_comp(sp, 3, off, u); store_comp(sp, 4, off, v); store_comp(sp, 5, off, w);
   store_comp(sp, 6, off, q);
-  *(SPIC_GLOBAL uint32_t *)((SPIC_GLOBAL char *)gmem(sp.id_out) + o * 4u) = pid;
+  // the particle's load index, in engines that keep the load order
+  if (sp.id_out) *(SPIC_GLOBAL uint32_t *)((SPIC_GLOBAL char *)gmem(sp.id_out) + o * 4u) = pid;
 }
 
 // Write a pushed particle to the output buffer.  A particle that stays in
@@ -881,8 +882,9 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
         const uint32_t off = i * (uint32_t)sizeof(T);
 #pragma unroll
         for (int c = 0; c < 7; ++c) nxt[c] = load_comp(sp, c, off);
-        nxt_id = *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) +
-                                                 i * 4u);
+        if (sp.id_in)
+          nxt_id = *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) +
+                                                   i * 4u);
       }
     };
     if (PREFETCH) load_particle(tid);
@@ -1151,12 +1153,6 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
   }
 }
 
-// Instance of block b in a batch map (uniform across the workgroup).
-__device__ __forceinline__ int batch_instance(const BatchMap &map, int b) {
-  int i = 0;
-  for (int k = 1; k < map.n; ++k) i += b >= map.start[k] ? 1 : 0;
-  return i;
-}
 
 // PUSH false: the deposit alone (spic_engine_deposit), in its own
 // instances so that the phase kernels carry no test for it.
@@ -1167,20 +1163,10 @@ void k_binned_phase(BinnedArgs<T> a) {
   binned_phase_tile<T, A, TX, TY, TZ, PUSH, FX>(a, blockIdx.x);
 }
 
-// Every instance of a batch in one grid: instance i owns blocks
-// [map.start[i], map.start[i + 1]), one per tile, with arguments tab[i].
-template <typename T, typename A, int TX, int TY, int TZ, bool FX = false>
-__global__ __launch_bounds__(4 * TX * TY * TZ)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
-void k_binned_phase_batch(const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
-  const int i = batch_instance(map, blockIdx.x);
-  binned_phase_tile<T, A, TX, TY, TZ, true, FX>(tab[i], blockIdx.x - map.start[i]);
-}
-
 // The particles of every species' input overflow region: global gathers
 // and deposits, re-binned like the tile kernel's.
-// Part `part` of `nparts` (workgroups of 256 threads) takes every
-// (nparts * 256)-th particle.
+// Part `part` of `nparts` (workgroups of blockDim.x threads) takes every
+// (nparts * blockDim.x)-th particle.
 template <typename T, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
 __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int part,
                                                      int nparts) {
@@ -1195,14 +1181,17 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
     const BinnedSpecies<T> &sp = a.sp[s];
     const SlotRange ov = slot_range(a, sp, a.ntiles);
     const uint32_t nin = ov.count;
-    for (uint32_t r = part * 256u + threadIdx.x; r < nin; r += (uint32_t)nparts * 256u) {
+    for (uint32_t r = part * blockDim.x + threadIdx.x; r < nin;
+         r += (uint32_t)nparts * blockDim.x) {
       const uint32_t i = ov.slot(r);
       const uint32_t off = i * (uint32_t)sizeof(T);
       T x = load_comp(sp, 0, off), y = load_comp(sp, 1, off), z = load_comp(sp, 2, off);
       T u = load_comp(sp, 3, off), v = load_comp(sp, 4, off), w = load_comp(sp, 5, off);
       const T q = load_comp(sp, 6, off);
       const uint32_t pid =
-          *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) + i * 4u);
+          sp.id_in
+              ? *(const SPIC_GLOBAL uint32_t *)((const SPIC_GLOBAL char *)gmem(sp.id_in) + i * 4u)
+              : 0u;
       int ix, iy, iz;
       if constexpr (FX) {
         int X = __float_as_int(x), Y = __float_as_int(y), Z = __float_as_int(z);
@@ -1252,13 +1241,107 @@ __global__ __launch_bounds__(256) void k_binned_overflow(BinnedArgs<T> a) {
   binned_overflow_part<T, TX, TY, TZ, PUSH, FX>(a, blockIdx.x, gridDim.x);
 }
 
-template <typename T, int TX, int TY, int TZ, bool FX = false>
-__global__ __launch_bounds__(256) void k_binned_overflow_batch(
-    const BinnedArgs<T> *__restrict__ tab, BatchMap map) {
-  const int i = batch_instance(map, blockIdx.x);
-  binned_overflow_part<T, TX, TY, TZ, true, FX>(tab[i], blockIdx.x - map.start[i],
-                                                map.start[i + 1] - map.start[i]);
+// Every instance's particle work of a batch phase in one persistent
+// launch: each workgroup takes the next item of the work list `items`
+// (spic_binned.hpp, QueueItem) from the device counter ctr[0] until the
+// list is exhausted, so the grid is sized to the workgroups the device holds
+// at once and the longest items, listed first, start first.  Items are
+// independent (every tile and overflow part reads the input buffers and
+// writes the outputs through counters), so no order among them is needed and
+// any grid size completes.  The last workgroup to leave (ctr[1]) resets both
+// counters for the next launch: a replayed graph needs no reset node.
+template <typename T, typename A, int TX, int TY, int TZ, bool FX = false>
+__global__ __launch_bounds__(4 * TX * TY * TZ)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+void k_binned_phase_queue(const BinnedArgs<T> *__restrict__ tab,
+                          const uint32_t *__restrict__ items, uint32_t nitems,
+                          unsigned int *ctr) {
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = gatomic_add(ctr, 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();  // every thread has read s_item before it is written again
+    if (it >= nitems) break;
+    const uint32_t w = gmem(items)[it];
+    const int inst = (int)(w >> kQueueInstShift);
+    const uint32_t idx = w & kQueueIndexMask;
+    if (w & kQueueOverflowBit)
+      binned_overflow_part<T, TX, TY, TZ, true, FX>(tab[inst], (int)idx, kQueueOverflowParts);
+    else
+      binned_phase_tile<T, A, TX, TY, TZ, true, FX>(tab[inst], (int)idx);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (gatomic_add(ctr + 1, 1u) == gridDim.x - 1) {
+      // every workgroup has taken its last item: no counter use remains
+      gmem(ctr)[0] = 0u;
+      gmem(ctr)[1] = 0u;
+      __threadfence();
+    }
+  }
 }
+
+template <typename T, typename A, int TX, int TY, int TZ, bool FX>
+hipError_t launch_queue_shape(const BinnedArgs<T> *tab, const uint32_t *items,
+                              uint32_t nitems, unsigned int *ctr, hipStream_t s) {
+  constexpr int NT = 4 * TX * TY * TZ;
+  // workgroups the device holds at once (occupancy x compute units), once
+  // per kernel instance
+  static int grid = 0;
+  if (grid == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err == hipSuccess)
+      err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (err == hipSuccess)
+      err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&k_binned_phase_queue<T, A, TX, TY, TZ, FX>),
+          NT, 0);
+    if (err != hipSuccess) return err;
+    grid = max(1, cus * max(1, per_cu));
+  }
+  const int blocks = (int)min((uint32_t)grid, nitems);
+  if (blocks < 1) return hipSuccess;
+  hipLaunchKernelGGL((k_binned_phase_queue<T, A, TX, TY, TZ, FX>), dim3(blocks), dim3(NT), 0, s,
+                     tab, items, nitems, ctr);
+  return hipGetLastError();
+}
+
+template <typename T, typename A, int TX, int TY, int TZ>
+hipError_t launch_queue_pos(const BinnedArgs<T> *tab, const uint32_t *items, uint32_t nitems,
+                            unsigned int *ctr, int fixed, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && !kStrictB) {
+    if (fixed) return launch_queue_shape<T, A, TX, TY, TZ, true>(tab, items, nitems, ctr, s);
+  } else {
+    if (fixed) return hipErrorInvalidValue;
+  }
+  return launch_queue_shape<T, A, TX, TY, TZ, false>(tab, items, nitems, ctr, s);
+}
+
+template <typename T>
+hipError_t launch_binned_phase_queue(const BinnedArgs<T> *tab, const uint32_t *items,
+                                     uint32_t nitems, unsigned int *ctr, int tx, int ty,
+                                     int tz, int lds_fp64, int fixed, hipStream_t s) {
+  if (nitems == 0) return hipSuccess;
+  const bool wide = sizeof(T) == 4 && lds_fp64;
+  if (tx == 4 && ty == 4 && tz == 4)
+    return wide ? launch_queue_pos<T, double, 4, 4, 4>(tab, items, nitems, ctr, fixed, s)
+                : launch_queue_pos<T, T, 4, 4, 4>(tab, items, nitems, ctr, fixed, s);
+  if (tx == 8 && ty == 4 && tz == 4)
+    return wide ? launch_queue_pos<T, double, 8, 4, 4>(tab, items, nitems, ctr, fixed, s)
+                : launch_queue_pos<T, T, 8, 4, 4>(tab, items, nitems, ctr, fixed, s);
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_binned_phase_queue<double>(const BinnedArgs<double> *,
+                                                      const uint32_t *, uint32_t,
+                                                      unsigned int *, int, int, int, int, int,
+                                                      hipStream_t);
+template hipError_t launch_binned_phase_queue<float>(const BinnedArgs<float> *,
+                                                     const uint32_t *, uint32_t,
+                                                     unsigned int *, int, int, int, int, int,
+                                                     hipStream_t);
 
 template <typename T, typename A, bool PUSH, bool FX>
 hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
@@ -1303,49 +1386,6 @@ hipError_t launch_binned_phase(const BinnedArgs<T> &a, hipStream_t s) {
 
 template hipError_t launch_binned_phase<double>(const BinnedArgs<double> &, hipStream_t);
 template hipError_t launch_binned_phase<float>(const BinnedArgs<float> &, hipStream_t);
-
-template <typename T, typename A, int TX, int TY, int TZ, bool FX>
-hipError_t launch_batch_shape(const BinnedArgs<T> *tab, const BatchMap &tiles,
-                              const BatchMap &ovf, hipStream_t s) {
-  hipLaunchKernelGGL((k_binned_phase_batch<T, A, TX, TY, TZ, FX>),
-                     dim3(tiles.start[tiles.n]), dim3(4 * TX * TY * TZ), 0, s, tab, tiles);
-  hipLaunchKernelGGL((k_binned_overflow_batch<T, TX, TY, TZ, FX>), dim3(ovf.start[ovf.n]),
-                     dim3(256), 0, s, tab, ovf);
-  return hipGetLastError();
-}
-
-template <typename T, typename A, int TX, int TY, int TZ>
-hipError_t launch_batch_pos(const BinnedArgs<T> *tab, const BatchMap &tiles,
-                            const BatchMap &ovf, int fixed, hipStream_t s) {
-  if constexpr (sizeof(T) == 4 && !kStrictB) {
-    if (fixed) return launch_batch_shape<T, A, TX, TY, TZ, true>(tab, tiles, ovf, s);
-  } else {
-    if (fixed) return hipErrorInvalidValue;
-  }
-  return launch_batch_shape<T, A, TX, TY, TZ, false>(tab, tiles, ovf, s);
-}
-
-template <typename T>
-hipError_t launch_binned_phase_batch(const BinnedArgs<T> *tab, const BatchMap &tiles,
-                                     const BatchMap &ovf, int tx, int ty, int tz,
-                                     int lds_fp64, int fixed, hipStream_t s) {
-  if (tiles.n <= 0 || tiles.start[tiles.n] <= 0) return hipSuccess;
-  const bool wide = sizeof(T) == 4 && lds_fp64;
-  if (tx == 4 && ty == 4 && tz == 4)
-    return wide ? launch_batch_pos<T, double, 4, 4, 4>(tab, tiles, ovf, fixed, s)
-                : launch_batch_pos<T, T, 4, 4, 4>(tab, tiles, ovf, fixed, s);
-  if (tx == 8 && ty == 4 && tz == 4)
-    return wide ? launch_batch_pos<T, double, 8, 4, 4>(tab, tiles, ovf, fixed, s)
-                : launch_batch_pos<T, T, 8, 4, 4>(tab, tiles, ovf, fixed, s);
-  return hipErrorInvalidValue;
-}
-
-template hipError_t launch_binned_phase_batch<double>(const BinnedArgs<double> *,
-                                                      const BatchMap &, const BatchMap &,
-                                                      int, int, int, int, int, hipStream_t);
-template hipError_t launch_binned_phase_batch<float>(const BinnedArgs<float> *,
-                                                     const BatchMap &, const BatchMap &,
-                                                     int, int, int, int, int, hipStream_t);
 
 }  // namespace SPIC_NS
 }  // namespace spic

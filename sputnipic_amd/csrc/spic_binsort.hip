@@ -214,7 +214,7 @@ __global__ void k_bin_gather(Arrays7<TS> src, const uint32_t *__restrict__ src_i
       }
       dst.p[a][o] = (T)src.p[a][j];
     }
-    dst_id[o] = src_id ? src_id[j] : j;
+    if (dst_id) dst_id[o] = src_id ? src_id[j] : j;
   }
 }
 
@@ -247,6 +247,7 @@ __global__ void k_bin_bounds(const uint32_t *__restrict__ keys, int shift, int64
 // the caller's element type TD; fixed-point positions converted back.
 template <typename T, typename TD>
 __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
+                            const uint32_t *__restrict__ base,
                             const int64_t *__restrict__ seg,
                             const unsigned int *__restrict__ cnt,
                             unsigned int ovf_cap, int ntiles, Arrays7<TD> dst, int fixed,
@@ -254,7 +255,7 @@ __global__ void k_bin_store(Arrays7<T> src, const uint32_t *__restrict__ id,
   const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
   for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
     const int64_t i = v.slot(r);
-    const uint32_t k = id[i];
+    const uint32_t k = id ? id[i] : base[blockIdx.x] + (uint32_t)r;
 #pragma unroll
     for (int a = 0; a < 7; ++a) {
       if constexpr (sizeof(T) == 4) {
@@ -278,13 +279,14 @@ __device__ __forceinline__ uint32_t be32(T v) {
 // float32: the datasets of a particle snapshot.
 template <typename T>
 __global__ void k_bin_store_be32(Arrays7<T> src, const uint32_t *__restrict__ id,
+                                 const uint32_t *__restrict__ base,
                                  const int64_t *__restrict__ seg,
                                  const unsigned int *__restrict__ cnt, unsigned int ovf_cap,
                                  int ntiles, Words6 dst, int fixed, FixedParams fx) {
   const RangeView v = range_view(seg, cnt, ntiles, ovf_cap, (int)blockIdx.x);
   for (int64_t r = threadIdx.x; r < v.count; r += blockDim.x) {
     const int64_t i = v.slot(r);
-    const uint32_t k = id[i];
+    const uint32_t k = id ? id[i] : base[blockIdx.x] + (uint32_t)r;
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
       if constexpr (sizeof(T) == 4) {
@@ -463,7 +465,7 @@ __device__ __forceinline__ void copy_particle(PS src, PSI src_id, int64_t stride
                                               PD dst, PDI dst_id, int64_t o) {
 #pragma unroll
   for (int a = 0; a < 7; ++a) dst[a * stride + o] = src[a * stride + j];
-  dst_id[o] = src_id[j];
+  if (dst_id) dst_id[o] = src_id[j];  // engines that keep no ids have neither
 }
 
 constexpr int kTileThreads = 512;
@@ -809,7 +811,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sort(
   const unsigned int n_own = s_own;
   T *stage = reinterpret_cast<T *>(s_raw);
   uint32_t *stage_id = reinterpret_cast<uint32_t *>(s_raw);
-  for (int a = 0; a < 8; ++a) {
+  const int ncomp = dst_id ? 8 : 7;  // the ids are the eighth, when kept
+  for (int a = 0; a < ncomp; ++a) {
     for (unsigned int c0 = 0; c0 < n_own; c0 += (unsigned)kChunk) {
       for (int64_t r0 = tid; r0 < v.count; r0 += kTileU * kTileThreads) {
         unsigned int p[kTileU];
@@ -1054,7 +1057,7 @@ template hipError_t bin_sort<float>(const KGrid<float> &, const BinLayout &, int
 
 template <typename T>
 hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
-                     const int64_t *seg, const unsigned int *cnt,
+                     const uint32_t *base, const int64_t *seg, const unsigned int *cnt,
                      unsigned int ovf_cap, void *const dst[7], int dst_f64, hipStream_t s) {
   Arrays7<T> as;
   for (int a = 0; a < 7; ++a) as.p[a] = src[a];
@@ -1062,46 +1065,76 @@ hipError_t bin_store(const BinLayout &L, T *const src[7], const uint32_t *id,
     Arrays7<double> ad;
     for (int a = 0; a < 7; ++a) ad.p[a] = (double *)dst[a];
     hipLaunchKernelGGL((k_bin_store<T, double>), dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
-                       seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
+                       base, seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
   } else {
     if (sizeof(T) == 8) return hipErrorInvalidValue;  // no narrowing store
     Arrays7<float> ad;
     for (int a = 0; a < 7; ++a) ad.p[a] = (float *)dst[a];
     hipLaunchKernelGGL((k_bin_store<T, float>), dim3(L.ntiles + 1), dim3(256), 0, s, as, id,
-                       seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
+                       base, seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t bin_store_be32(const BinLayout &L, T *const src[7], const uint32_t *id,
-                          const int64_t *seg, const unsigned int *cnt, unsigned int ovf_cap,
-                          uint32_t *const dst[6], hipStream_t s) {
+                          const uint32_t *base, const int64_t *seg, const unsigned int *cnt,
+                          unsigned int ovf_cap, uint32_t *const dst[6], hipStream_t s) {
   Arrays7<T> as;
   Words6 ad;
   for (int a = 0; a < 7; ++a) as.p[a] = src[a];
   for (int a = 0; a < 6; ++a) ad.p[a] = dst[a];
-  hipLaunchKernelGGL(k_bin_store_be32<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id, seg,
-                     cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
+  hipLaunchKernelGGL(k_bin_store_be32<T>, dim3(L.ntiles + 1), dim3(256), 0, s, as, id, base,
+                     seg, cnt, ovf_cap, L.ntiles, ad, L.fixed, L.fx);
   return hipGetLastError();
 }
 
 template hipError_t bin_store_be32<double>(const BinLayout &, double *const[7],
-                                           const uint32_t *, const int64_t *,
+                                           const uint32_t *, const uint32_t *, const int64_t *,
                                            const unsigned int *, unsigned int,
                                            uint32_t *const[6], hipStream_t);
 template hipError_t bin_store_be32<float>(const BinLayout &, float *const[7],
-                                          const uint32_t *, const int64_t *,
+                                          const uint32_t *, const uint32_t *, const int64_t *,
                                           const unsigned int *, unsigned int,
                                           uint32_t *const[6], hipStream_t);
 
 template hipError_t bin_store<double>(const BinLayout &, double *const[7],
-                                      const uint32_t *, const int64_t *,
+                                      const uint32_t *, const uint32_t *, const int64_t *,
                                       const unsigned int *, unsigned int,
                                       void *const[7], int, hipStream_t);
 template hipError_t bin_store<float>(const BinLayout &, float *const[7],
-                                     const uint32_t *, const int64_t *,
+                                     const uint32_t *, const uint32_t *, const int64_t *,
                                      const unsigned int *, unsigned int,
                                      void *const[7], int, hipStream_t);
+
+// Dense position of each range's first particle (tiles in order, then the
+// overflow region), for stores of an engine that keeps no ids: range counts
+// then an exclusive scan.  Workspace: counts | bases | rocPRIM storage; with
+// workspace == nullptr only *needed is set.  *bases points into the
+// workspace.
+hipError_t range_bases(const int64_t *seg, const unsigned int *cnt, unsigned int ovf_cap,
+                       int ntiles, void *workspace, size_t workspace_bytes, size_t *needed,
+                       const uint32_t **bases, hipStream_t s) {
+  const size_t n = (size_t)ntiles + 1;
+  const size_t a = align_up(n * sizeof(unsigned int));
+  size_t tmp = 0;
+  hipError_t e = rocprim::exclusive_scan(nullptr, tmp, (const unsigned int *)nullptr,
+                                         (unsigned int *)nullptr, 0u, n,
+                                         rocprim::plus<unsigned int>(), s);
+  if (e != hipSuccess) return e;
+  if (needed) *needed = 2 * a + align_up(tmp);
+  if (!workspace) return hipSuccess;
+  if (workspace_bytes < 2 * a + align_up(tmp)) return hipErrorInvalidValue;
+  unsigned int *counts = (unsigned int *)workspace;
+  unsigned int *out = (unsigned int *)((char *)workspace + a);
+  hipLaunchKernelGGL(k_range_counts, dim3(nblocks((long)n)), dim3(256), 0, s, seg,
+                     cnt, ovf_cap, ntiles, counts);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = rocprim::exclusive_scan((char *)workspace + 2 * a, tmp, counts, out, 0u, n,
+                              rocprim::plus<unsigned int>(), s);
+  if (e != hipSuccess) return e;
+  *bases = out;
+  return hipSuccess;
+}
 
 }  // namespace spic

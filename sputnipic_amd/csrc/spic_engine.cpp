@@ -3,6 +3,7 @@
 // Host code only; the kernels are in spic_binned.inl and spic_binsort.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <mutex>
@@ -27,6 +28,8 @@ struct spic_engine {
   int sort_mode = SPIC_SORT_TILE;  // re-sorts: tile_sort or bin_sort
   int lds_fp64 = 1;
   int set_moments = 0;             // phases write the moments (else add)
+  int keep_ids = 1;                // a load index per particle (store in load order)
+  int ovf_pct = 100;               // overflow slots per buffer, percent of n
   int64_t phases = 0, sorts = 0;
   uint64_t gen = 0;                // bumped by every load (storage may move)
   size_t bytes = 0;
@@ -38,7 +41,7 @@ struct spic_engine {
     int64_t slack = 0;             // free slots per segment after a sort
     int64_t cap = 0;               // slots per buffer
     int64_t alloc_cap = 0;         // slots the buffers were allocated for
-    unsigned int ovf_cap = 0;      // overflow slots (= n: nothing is ever lost)
+    unsigned int ovf_cap = 0;      // overflow slots (n by default: nothing is lost)
     void *block[2] = {};           // two buffers of x y z u v w q, one block each
     void *buf[2][7] = {};          // the components: block[b] + c * cap
     uint32_t *id[2] = {nullptr, nullptr};
@@ -313,6 +316,22 @@ hipError_t launch_phase(spic_engine *e, const spic_field *field,
   return hipSuccess;
 }
 
+// Dense store positions of species `is`'s ranges when the engine keeps no
+// ids (*bases stays null when it does): range counts and a scan into the
+// sort workspace.
+hipError_t store_bases(spic_engine *e, int is, const uint32_t **bases, hipStream_t s) {
+  *bases = nullptr;
+  const auto &sp = e->sp[is];
+  if (sp.id[sp.cur]) return hipSuccess;
+  size_t need = 0;
+  hipError_t err = spic::range_bases(sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, e->L.ntiles,
+                                     nullptr, 0, &need, bases, s);
+  if (err != hipSuccess) return err;
+  if ((err = ensure_workspace(e, need, s)) != hipSuccess) return err;
+  return spic::range_bases(sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, e->L.ntiles, e->ws,
+                           e->ws_bytes, nullptr, bases, s);
+}
+
 // A species loaded with n_sub_cycles != 1 runs only while a sub-cycling
 // mode is chosen (the mode may have changed since the load).
 int check_subcycles(const spic_engine *e) {
@@ -349,7 +368,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
   int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0,
-      sort_mode = SPIC_SORT_TILE, positions = 0;
+      sort_mode = SPIC_SORT_TILE, positions = 0, keep_ids = 1, ovf_pct = 100;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -371,6 +390,12 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
         opts->positions != SPIC_POS_FIXED)
       return fail(SPIC_EINVAL, "positions must be 0, SPIC_POS_FLOAT or SPIC_POS_FIXED");
     positions = opts->positions;
+    if (opts->order != 0 && opts->order != SPIC_ORDER_LOAD && opts->order != SPIC_ORDER_ENGINE)
+      return fail(SPIC_EINVAL, "order must be 0, SPIC_ORDER_LOAD or SPIC_ORDER_ENGINE");
+    keep_ids = opts->order == SPIC_ORDER_ENGINE ? 0 : 1;
+    if (opts->overflow_pct < 0 || opts->overflow_pct > 100)
+      return fail(SPIC_EINVAL, "overflow_pct must be 0..100");
+    if (opts->overflow_pct) ovf_pct = opts->overflow_pct;
   }
   if (positions == 0)
     positions = (grd->prec == SPIC_F32 && !spic::strict_mode()) ? SPIC_POS_FIXED
@@ -396,6 +421,8 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->lds_fp64 = lds_fp64;
   e->slack_pct = slack_pct;
   e->set_moments = set_mom;
+  e->keep_ids = keep_ids;
+  e->ovf_pct = ovf_pct;
   auto &L = e->L;
   L.ncx = grd->nxc - 2; L.ncy = grd->nyc - 2; L.ncz = grd->nzc - 2;
   L.tx = tx; L.ty = ty; L.tz = tz;
@@ -482,8 +509,17 @@ int spic_engine_load_as(spic_engine *e, int is, const spic_species *src, int pre
   const int64_t nt = e->L.ntiles;
   int64_t slack = (n * e->slack_pct + 100 * nt - 1) / (100 * nt);
   if (slack < 16) slack = 16;
-  // segments (n + nt * slack) then the overflow region (n)
-  const int64_t cap = 2 * n + nt * slack;
+  // segments (n + nt * slack) then the overflow region: n slots by default
+  // (no particle can be lost), else overflow_pct percent of n (at least
+  // 1024 slots), and a phase whose particles do not fit reports SPIC_ENOMEM
+  int64_t ovf = n;
+  if (e->ovf_pct < 100) {
+    ovf = (n * e->ovf_pct + 99) / 100;
+    if (ovf < 1024) ovf = 1024;
+    if (ovf > n) ovf = n;
+  }
+  if (ovf < 1) ovf = 1;
+  const int64_t cap = n + nt * slack + ovf;
   // the phase kernels address a slot by its 32-bit byte offset within a
   // component array
   if ((uint64_t)cap * (uint64_t)e->esz > 0xffffffffull)
@@ -505,7 +541,8 @@ int spic_engine_load_as(spic_engine *e, int is, const spic_species *src, int pre
     for (int b = 0; b < 2 && !what; ++b) {
       if ((err = dmalloc(e, &sp.block[b], (size_t)e->esz * 7 * cap)) != hipSuccess)
         what = "engine particle storage";
-      else if ((err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
+      else if (e->keep_ids &&
+               (err = dmalloc(e, (void **)&sp.id[b], sizeof(uint32_t) * cap)) != hipSuccess)
         what = "engine id storage";
       else if ((err = dmalloc(e, (void **)&sp.cnt[b],
                               sizeof(unsigned int) * 3 * (nt + 1))) != hipSuccess ||
@@ -524,7 +561,7 @@ int spic_engine_load_as(spic_engine *e, int is, const spic_species *src, int pre
   sp.n = n;
   sp.cap = cap;
   sp.slack = slack;
-  sp.ovf_cap = (unsigned int)(n > 0 ? n : 1);
+  sp.ovf_cap = (unsigned int)ovf;
   sp.qom = src->qom;
   sp.niter = src->NiterMover;
   sp.nsub = src->n_sub_cycles;
@@ -566,12 +603,16 @@ int spic_engine_store_as(spic_engine *e, int is, spic_species *dst, int prec,
   hipStream_t s = (hipStream_t)stream;
   void *dsts[7] = {dst->x, dst->y, dst->z, dst->u, dst->v, dst->w, dst->q};
   const int f64 = prec == SPIC_F64 ? 1 : 0;
-  hipError_t err =
-      e->esz == 8
-          ? spic::bin_store<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                    sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, dsts, f64, s)
-          : spic::bin_store<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                   sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, dsts, f64, s);
+  const uint32_t *bases = nullptr;
+  hipError_t err = store_bases(e, is, &bases, s);
+  if (err == hipSuccess)
+    err = e->esz == 8
+              ? spic::bin_store<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                        bases, sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
+                                        dsts, f64, s)
+              : spic::bin_store<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
+                                       bases, sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap,
+                                       dsts, f64, s);
   if (err == hipSuccess && !spic::async_mode()) err = hipStreamSynchronize(s);
   return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine store");
 }
@@ -592,12 +633,16 @@ int spic_engine_snapshot_be32(spic_engine *e, int is, void *const dst[6], void *
   auto &sp = e->sp[is];
   hipStream_t s = (hipStream_t)stream;
   uint32_t *const *d = (uint32_t *const *)dst;
-  hipError_t err =
-      e->esz == 8
-          ? spic::bin_store_be32<double>(e->L, (double *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                         sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, d, s)
-          : spic::bin_store_be32<float>(e->L, (float *const *)sp.buf[sp.cur], sp.id[sp.cur],
-                                        sp.seg[sp.segi], sp.cnt[sp.cur], sp.ovf_cap, d, s);
+  const uint32_t *bases = nullptr;
+  hipError_t err = store_bases(e, is, &bases, s);
+  if (err == hipSuccess)
+    err = e->esz == 8
+              ? spic::bin_store_be32<double>(e->L, (double *const *)sp.buf[sp.cur],
+                                             sp.id[sp.cur], bases, sp.seg[sp.segi],
+                                             sp.cnt[sp.cur], sp.ovf_cap, d, s)
+              : spic::bin_store_be32<float>(e->L, (float *const *)sp.buf[sp.cur],
+                                            sp.id[sp.cur], bases, sp.seg[sp.segi],
+                                            sp.cnt[sp.cur], sp.ovf_cap, d, s);
   if (err == hipSuccess && !spic::async_mode()) err = hipStreamSynchronize(s);
   return err == hipSuccess ? SPIC_OK : hip_fail(err, "engine snapshot");
 }
@@ -731,7 +776,7 @@ struct spic_batch {
   // state: table (segi * 2 + cur)
   void *tab = nullptr;
   size_t arg_bytes = 0;
-  spic::BatchMap tiles{}, ovf{}, nodes{}, counts{};
+  spic::BatchMap nodes{}, counts{};
   int max_ns = 0;
   // re-sorts of the whole batch in four launches when every member sorts
   // by tile sort: device tables of the jobs (member species with
@@ -743,11 +788,75 @@ struct spic_batch {
   spic::TileSortMap sort_map{};
   unsigned int *sort_ws = nullptr;
   size_t sort_ws_words = 0, sort_ws_cap = 0;
+  // the persistent phase launch: its work list (device; tiles of every
+  // instance, heaviest first, then the overflow parts) and its two counters
+  // (device, zero between launches)
+  uint32_t *items = nullptr;
+  size_t items_cap = 0;
+  uint32_t nitems = 0;
+  unsigned int *queue_ctr = nullptr;
 };
 
 namespace {
 
-constexpr int kBatchOverflowBlocks = 64;  // overflow workgroups per instance
+constexpr int kBatchOverflowBlocks = spic::kQueueOverflowParts;  // per instance
+
+// The work list of the persistent phase launch: every tile of every
+// instance with particles, in decreasing order of the particles its
+// segments held at the last sort (all species), so that the heaviest tiles
+// start first; then each instance's overflow parts.
+hipError_t build_items(spic_batch *b) {
+  std::vector<std::pair<int64_t, uint32_t>> tiles;
+  std::vector<uint32_t> items;
+  std::vector<int64_t> seg;
+  for (int i = 0; i < b->n; ++i) {
+    const spic_engine *e = b->e[i];
+    const int nt = e->L.ntiles;
+    std::vector<int64_t> load(nt, 0);
+    bool any = false;
+    for (int is = 0; is < e->ns; ++is) {
+      const auto &sp = e->sp[is];
+      if (!sp.loaded || sp.n == 0) continue;
+      any = true;
+      seg.resize((size_t)nt + 1);
+      hipError_t err = hipMemcpy(seg.data(), sp.seg[sp.segi], sizeof(int64_t) * (nt + 1),
+                                 hipMemcpyDeviceToHost);
+      if (err != hipSuccess) return err;
+      for (int t = 0; t < nt; ++t) load[t] += seg[t + 1] - seg[t];
+    }
+    if (!any) continue;
+    for (int t = 0; t < nt; ++t)
+      tiles.emplace_back(load[t], ((uint32_t)i << spic::kQueueInstShift) | (uint32_t)t);
+    for (int p = 0; p < kBatchOverflowBlocks; ++p)
+      items.push_back(((uint32_t)i << spic::kQueueInstShift) | spic::kQueueOverflowBit |
+                      (uint32_t)p);
+  }
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const std::pair<int64_t, uint32_t> &a,
+                      const std::pair<int64_t, uint32_t> &c) { return a.first > c.first; });
+  std::vector<uint32_t> all;
+  all.reserve(tiles.size() + items.size());
+  for (const auto &t : tiles) all.push_back(t.second);
+  all.insert(all.end(), items.begin(), items.end());
+  hipError_t err;
+  if (all.size() > b->items_cap) {
+    if (b->items) (void)hipFree(b->items);
+    b->items = nullptr;
+    b->items_cap = 0;
+    if ((err = hipMalloc(&b->items, sizeof(uint32_t) * all.size())) != hipSuccess) return err;
+    b->items_cap = all.size();
+  }
+  if (!b->queue_ctr) {
+    if ((err = hipMalloc(&b->queue_ctr, 2 * sizeof(unsigned int))) != hipSuccess) return err;
+    if ((err = hipMemset(b->queue_ctr, 0, 2 * sizeof(unsigned int))) != hipSuccess) return err;
+  }
+  if (!all.empty() &&
+      (err = hipMemcpy(b->items, all.data(), sizeof(uint32_t) * all.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+    return err;
+  b->nitems = (uint32_t)all.size();
+  return hipSuccess;
+}
 
 template <typename T>
 hipError_t build_tables(spic_batch *b, unsigned int *status) {
@@ -769,14 +878,12 @@ hipError_t build_tables(spic_batch *b, unsigned int *status) {
     return err;
   b->arg_bytes = nb;
   auto blocks = [](int64_t items, int per) { return (int)((items + per - 1) / per); };
-  b->tiles.n = b->ovf.n = b->nodes.n = b->counts.n = b->n;
-  b->tiles.start[0] = b->ovf.start[0] = b->nodes.start[0] = b->counts.start[0] = 0;
+  b->nodes.n = b->counts.n = b->n;
+  b->nodes.start[0] = b->counts.start[0] = 0;
   for (int i = 0; i < b->n; ++i) {
     const spic_engine *e = b->e[i];
     const bool any = host[i].ns > 0;
     const int64_t nn = (int64_t)e->grd.nxn * e->grd.nyn * e->grd.nzn;
-    b->tiles.start[i + 1] = b->tiles.start[i] + (any ? e->L.ntiles : 0);
-    b->ovf.start[i + 1] = b->ovf.start[i] + (any ? kBatchOverflowBlocks : 0);
     b->nodes.start[i + 1] = b->nodes.start[i] + (any ? blocks(nn, 256) : 0);
     const int64_t ncnt = 3 * (int64_t)(e->L.ntiles + 1);
     const int64_t zeroed =
@@ -785,6 +892,7 @@ hipError_t build_tables(spic_batch *b, unsigned int *status) {
     b->gen[i] = e->gen;
   }
   b->status = status;
+  if ((err = build_items(b)) != hipSuccess) return err;
 
   b->tile_sorts = true;
   for (int i = 0; i < b->n; ++i)
@@ -879,11 +987,14 @@ hipError_t launch_batch(spic_batch *b, int state, hipStream_t s) {
   hipError_t err = spic::launch_batch_zero_counts<T>(tab, b->counts, b->max_ns, set, s);
   if (err == hipSuccess && set) err = zero_empty_batch_moments(b, s);
   if (err != hipSuccess) return err;
+  // every instance's tiles and overflow parts in one persistent launch
   err = spic::strict_mode()
-            ? spic::spic_strict::launch_binned_phase_batch<T>(
-                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, L.fixed, s)
-            : spic::spic_fast::launch_binned_phase_batch<T>(
-                  tab, b->tiles, b->ovf, L.tx, L.ty, L.tz, b->e[0]->lds_fp64, L.fixed, s);
+            ? spic::spic_strict::launch_binned_phase_queue<T>(
+                  tab, b->items, b->nitems, b->queue_ctr, L.tx, L.ty, L.tz,
+                  b->e[0]->lds_fp64, L.fixed, s)
+            : spic::spic_fast::launch_binned_phase_queue<T>(
+                  tab, b->items, b->nitems, b->queue_ctr, L.tx, L.ty, L.tz,
+                  b->e[0]->lds_fp64, L.fixed, s);
   if (err != hipSuccess) return err;
   return spic::launch_batch_reduce<T>(tab, b->nodes, b->max_ns, L.tx, L.ty, L.tz, s);
 }
@@ -1051,7 +1162,9 @@ int spic_batch_phase(spic_batch *b, void *stream) {
 
 int spic_batch_destroy(spic_batch *b) {
   if (!b) return SPIC_OK;
-  if (b->tab || b->sort_tab || b->sort_ws) (void)hipDeviceSynchronize();
+  if (b->tab || b->sort_tab || b->sort_ws || b->items) (void)hipDeviceSynchronize();
+  if (b->items) (void)hipFree(b->items);
+  if (b->queue_ctr) (void)hipFree(b->queue_ctr);
   if (b->tab) (void)hipFree(b->tab);
   if (b->sort_tab) (void)hipFree(b->sort_tab);
   if (b->sort_ws) (void)hipFree(b->sort_ws);
