@@ -58,6 +58,49 @@ def measured_traffic(prec: str):
         return None
 
 
+class EngineRunner:
+    """A fresh engine of the 262M-particle Harris sheet with a given tile
+    shape and sort interval, for the autotuner (sputnipic_amd.tune)."""
+
+    def __init__(self, prec: str, tile, sort_every: int):
+        import sputnipic_amd as S
+        self.grd = S.Grid(*GRID, *BOX, prec)
+        gem = S.GemParams()
+        self.field = S.gem_fields(self.grd, gem)
+        self.eng = S.Engine(self.grd, NS, tile=tile, sort_every=sort_every, set_moments=True)
+        for s, sp in enumerate(S.harris_species(PPC)):
+            p = S.gem_particles(self.grd, gem, sp, s, seed=0)
+            self.eng.load(s, p)
+            for d in p.d.values():
+                d.free()
+        self.moms = [S.InterpDensSpecies(self.grd) for _ in range(NS)]
+        self.param = S.Parameters(dt=0.25, c=1.0, NiterMover=3)
+
+    def phase(self, stream) -> None:
+        self.eng.phase(self.field, self.moms, self.param, stream)
+
+    def close(self) -> None:
+        self.eng.close()
+        for m in self.moms:
+            for d in m.d.values():
+                d.free()
+        for d in self.field.d.values():
+            d.free()
+
+
+def tune_form(prec: str) -> dict:
+    """Tile shape and sort interval of the Harris-sheet engine, chosen by the
+    autotuner on the workload itself (each candidate a fresh engine, 1
+    warm-up and 10 timed phases)."""
+    from sputnipic_amd.tune import tune_engine
+    t0 = time.perf_counter()
+    res = tune_engine(lambda tile, se: EngineRunner(prec, tile, se), phases=10, warmup=1)
+    log(f"{prec}: tuned in {time.perf_counter() - t0:.1f} s: " + ", ".join(
+        f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}: {t.ms_per_phase:.3f} ms"
+        for t in res.trials))
+    return res.as_dict()
+
+
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
              tile, order: str = "load") -> dict:
     import ctypes as C
@@ -937,6 +980,9 @@ def main() -> int:
     ap.add_argument("--order", choices=("load", "engine"), default="load",
                     help="engine particle order of the timed forms: load (a load index per "
                          "particle, store in load order; default) or engine (no index)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the forms with --tile and --sort-every instead of the "
+                         "autotuner's choice")
     ap.add_argument("--no-engine-order", action="store_true",
                     help="skip the extra timing of both forms without the load index")
     ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=0,
@@ -1060,17 +1106,28 @@ def main() -> int:
         print(json.dumps(line), flush=True)
         return 0
     forms = {}
+    tuning = {}
+    settings = {}
     for prec in args.forms.split(","):
         tile = tuple(int(t) for t in args.tile.split("x"))
-        forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, args.sort_every,
+        sort_every = args.sort_every
+        if args.engine == "tiled" and not args.no_tune:
+            with _StdoutToStderr():
+                tuning[prec] = tune_form(prec)
+            tile = tuple(int(t) for t in tuning[prec]["tile"].split("x"))
+            sort_every = tuning[prec]["sort_every"]
+        settings[prec] = (tile, sort_every)
+        forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, sort_every,
                                tile, args.order)
+        forms[prec]["tile"] = "x".join(map(str, tile))
     # the same forms without the per-particle load index (reported beside the
     # headline, which keeps the default engine)
     forms_engine_order = {}
     if args.engine == "tiled" and args.order == "load" and not args.no_engine_order:
         for prec in args.forms.split(","):
+            tile, sort_every = settings[prec]
             forms_engine_order[prec] = run_form(prec, args.steps, args.warmup, args.engine,
-                                                args.sort_every, tile, "engine")
+                                                sort_every, tile, "engine")
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
     cpu = refleg = None
@@ -1108,6 +1165,9 @@ def main() -> int:
             "form": "plain FP64" if primary == "f64" else "FP32",
             "particles": f["particles"],
             "cycles_timed": args.steps,
+            "tile": f.get("tile"),
+            "sort_every": f.get("sort_every"),
+            "settings_from": "autotuner" if tuning else "command line",
         },
         "roofline": {
             "bound": "hbm",
@@ -1126,6 +1186,7 @@ def main() -> int:
         "device_copy": copy,
         "forms": {k: {kk: vv for kk, vv in v.items()} for k, v in forms.items()},
         "forms_engine_order": forms_engine_order or None,
+        "tuning": tuning or None,
         "cpu_baseline": cpu,
         "field_solve": field,
         "input_order": order,

@@ -1,7 +1,11 @@
-"""Autotuner of the batched particle phase: picks the tile shape (cells, and
-so threads, per workgroup: 4x4x4 = 256 or 8x4x4 = 512) and the sort
-schedule (rebuild the tile segments every N phases, or only on load) by
-timing each candidate on the caller's own workload.
+"""Autotuner of the particle phase: picks the tile shape (cells, and so
+threads, per workgroup: 4x4x4 = 256 or 8x4x4 = 512) and the sort schedule
+(rebuild the tile segments every N phases, or only on load) by timing each
+candidate on the caller's own workload -- a batch of independent instances
+(tune_batch) or one engine (tune_engine).  Particles per thread are one per
+batch: the LDS budget of the tile (field stage, FP64 moment block, per-cell
+buckets) holds exactly one batch of buckets at three workgroups per compute
+unit, so the workgroup shape is the tuned launch dimension.
 
 Each tile shape needs its own engines, so the caller supplies a builder:
 ``make_instances(tile)`` returns fresh instances (loaded from the same
@@ -108,3 +112,56 @@ def tune_batch(make_instances: Callable[[Tuple[int, int, int]], Sequence],
                 x.close()
     best = min(trials, key=lambda t: t.ms_per_phase)
     return TuneResult(best.tile, best.sort_every, best.ms_per_phase, trials)
+
+
+@dataclass
+class EngineTuneResult:
+    tile: Tuple[int, int, int]
+    sort_every: int
+    ms_per_phase: float
+    trials: List[Trial] = field(default_factory=list)
+
+    def as_dict(self) -> dict:
+        return TuneResult(self.tile, self.sort_every, self.ms_per_phase,
+                          self.trials).as_dict()
+
+
+def tune_engine(make_runner: Callable[[Tuple[int, int, int], int], object],
+                tiles: Sequence[Tuple[int, int, int]] = TILES,
+                sort_choices: Sequence[int] = SORT_CHOICES, phases: int = 10,
+                warmup: int = 1, stream: Optional[Stream] = None) -> EngineTuneResult:
+    """Tile shape and sort interval of one engine's phase.
+
+    ``make_runner(tile, sort_every)`` builds a fresh engine with those
+    settings, loaded from the caller's initial condition, and returns an
+    object with ``phase(stream)`` (one engine phase, the engine's own
+    sorts included) and ``close()``.  Each candidate runs ``warmup`` untimed
+    and ``phases`` timed phases on one stream in async mode, timed with HIP
+    events; the fastest wins."""
+    if not tiles or not sort_choices:
+        raise ValueError("tune_engine needs at least one tile shape and one sort choice")
+    if phases < 1 or warmup < 0:
+        raise ValueError("tune_engine needs phases >= 1 and warmup >= 0")
+    stream = stream or Stream()
+    trials: List[Trial] = []
+    for tile in tiles:
+        for se in sort_choices:
+            run = make_runner(tuple(tile), int(se))
+            was_async = get_async()
+            set_async(True)
+            try:
+                for _ in range(warmup):
+                    run.phase(stream)
+                a, b = Event(), Event()
+                a.record(stream)
+                for _ in range(phases):
+                    run.phase(stream)
+                b.record(stream)
+                stream.synchronize()
+                check_domain(stream)
+            finally:
+                set_async(was_async)
+                run.close()
+            trials.append(Trial(tuple(tile), int(se), a.elapsed_ms(b) / phases))
+    best = min(trials, key=lambda t: t.ms_per_phase)
+    return EngineTuneResult(best.tile, best.sort_every, best.ms_per_phase, trials)
