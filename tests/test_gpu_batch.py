@@ -304,3 +304,62 @@ def test_autotuner_times_every_candidate_and_picks_the_fastest():
     assert all(t.ms_per_phase > 0 for t in res.trials)
     d = res.as_dict()
     assert d["tile"] in ("4x4x4", "8x4x4") and len(d["trials"]) == 4
+
+
+def test_engine_autotuner_times_every_candidate_and_picks_the_fastest():
+    """tune_engine builds a fresh engine per (tile, sort interval), times
+    every one, closes each, and returns the fastest; the engines it times
+    advance the same particles as an engine outside it (checked on the last
+    candidate's phases against the oracle)."""
+    from sputnipic_amd.tune import tune_engine
+    st = _oracle(INSTANCES[0], "f64")
+    c = st.cfg
+    built, closed, phases = [], [], {}
+
+    class Runner:
+        def __init__(self, tile, se):
+            self.key = (tile, se)
+            built.append(self.key)
+            self.grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f64")
+            self.field = S.EMField(self.grd, st.fields)
+            self.parts = [S.Particles(p, c.qom[s], "f64", c.NiterMover)
+                          for s, p in enumerate(st.parts)]
+            self.ids = [S.InterpDensSpecies(self.grd) for _ in range(c.ns)]
+            self.param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover,
+                                      PERIODICX=c.PERIODICX, PERIODICY=c.PERIODICY,
+                                      PERIODICZ=c.PERIODICZ)
+            self.eng = S.Engine(self.grd, c.ns, tile=tile, sort_every=se, set_moments=True)
+            for s, p in enumerate(self.parts):
+                self.eng.load(s, p)
+
+        def phase(self, stream):
+            self.eng.phase(self.field, self.ids, self.param, stream)
+            phases[self.key] = phases.get(self.key, 0) + 1
+
+        def close(self):
+            for s, p in enumerate(self.parts):
+                self.eng.store(s, p)
+            self.out = [p.download() for p in self.parts]
+            self.eng.close()
+            closed.append(self.key)
+            last[0] = self
+
+    last = [None]
+    res = tune_engine(lambda tile, se: Runner(tile, se), tiles=((4, 4, 4), (8, 4, 4)),
+                      sort_choices=(0, 2), phases=3, warmup=1)
+    assert built == closed == [((4, 4, 4), 0), ((4, 4, 4), 2), ((8, 4, 4), 0),
+                               ((8, 4, 4), 2)]
+    assert all(n == 4 for n in phases.values())
+    assert len(res.trials) == 4 and all(t.ms_per_phase > 0 for t in res.trials)
+    best = min(res.trials, key=lambda t: t.ms_per_phase)
+    assert (res.tile, res.sort_every) == (best.tile, best.sort_every)
+    assert res.as_dict()["tile"] in ("4x4x4", "8x4x4")
+    # the last candidate (8x4x4, a re-sort every 2 phases) ran 4 phases:
+    # its particles are the oracle's after 4 cycles, within production
+    # arithmetic's bounds
+    for _ in range(4):
+        st.cycle()
+    for s in range(c.ns):
+        for k in ("u", "v", "w"):
+            got, want = last[0].out[s][k], st.parts[s][k]
+            assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), (s, k)
