@@ -73,6 +73,14 @@ __device__ __forceinline__ int bcell_floor(T p, double start, T inv, int n) {
   }
 }
 
+// s_waitcnt vmcnt(0) (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).  Ends
+// the rare global-memory stencil gather inside its own branch.  Without it
+// the stencil registers, which the common branch fills from LDS, reach the
+// interpolation still counted as vector-memory loads, and the compiler
+// waits there for every vector memory operation in flight: the next
+// batch's prefetched particles and the previous batch's stores.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Stencil indices must address nodes 0..n-1: ix-1 >= 0 and ix <= n-1.
 __device__ __forceinline__ bool bclamp(int &ix, int n) {
   bool bad = (ix < 1) | (ix > n - 1);
@@ -207,6 +215,7 @@ __device__ __forceinline__ void push_particle_once(const KGrid<T> &g, const KFie
 #pragma unroll
               for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = gp[c][n];
             }
+        if constexpr (STAGED) wait_vm_all();
       }
     }
     const T xi[2] = {xc - X0, X1 - xc};
@@ -404,6 +413,7 @@ __device__ __forceinline__ void push_fixed_once(const KGrid<float> &g, const KFi
 #pragma unroll
               for (int c = 0; c < 6; ++c) F[aa][bb][cc][c] = gp[c][n];
             }
+        if constexpr (STAGED) wait_vm_all();
       }
     }
     const float ox = (float)ox_i * fp.unit[0];
@@ -867,10 +877,15 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
     const uint32_t nbatches = (nin + NT - 1) / NT;
-    // this thread's particle of batch j (x y z u v w q and its id); with
-    // PREFETCH (FP32, which has the registers for it at three waves per
-    // SIMD; FP64 would spill) batch j + 1 is loaded while batch j is pushed
-    constexpr bool PREFETCH = sizeof(T) == 4;
+    // this thread's particle of batch j (x y z u v w q and its id).  With
+    // PREFETCH batch j + 1 is loaded while batch j is processed: FP32, which
+    // has the registers for it at three waves per SIMD, and the deposit
+    // alone (8.2 -> 7.3 ms per FP64 deposit of the 262M-particle sheet).
+    // The FP64 phase kernel loads each batch as it starts: its push needs
+    // all 256 VGPRs, and loading batch j + 1 during the push (spills) or
+    // after it (in flight over the bucket and list steps) measured 2-5%
+    // slower.
+    constexpr bool PREFETCH = sizeof(T) == 4 || !PUSH;
     T nxt[7];
     uint32_t nxt_id = 0;
     auto load_particle = [&](uint32_t r) {
