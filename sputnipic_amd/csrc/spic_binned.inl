@@ -858,7 +858,10 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
   const int my_a = (tid >> 1) & 1, my_b = tid & 1;
   const int mlx = my_cell / (TY * TZ), mly = (my_cell / TZ) % TY, mlz = my_cell % TZ;
 
-  bool bad_move = false, bad_dep = false, bad_store = false;
+  // per-lane error flags (bit 0: move, 1: deposit cell, 2: store), an int
+  // so that the loops carry them in a VGPR: carried bools become 64-bit
+  // lane masks in scalar registers, which the FP64 kernel spills
+  int bad_bits = 0;
   unsigned int n_gg = 0, n_gd = 0, n_lg = 0, n_ld = 0;
 
   for (int s = 0; s < a.ns; ++s) {
@@ -919,6 +922,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       const uint32_t r = j * NT + tid;
       const bool active = r < nin;
       const int buf = (int)(j & 1);
+      bool bad_move = false, bad_dep = false, bad_store = false;  // this batch's
       if (!PREFETCH) load_particle(r);
       T x = nxt[0], y = nxt[1], z = nxt[2], u = nxt[3], v = nxt[4], w = nxt[5], q = nxt[6];
       uint32_t pid = nxt_id;
@@ -1132,6 +1136,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           lst_done = nl;
         }
       }
+      bad_bits |= (bad_move ? 1 : 0) | (bad_dep ? 2 : 0) | (bad_store ? 4 : 0);
       __syncthreads();
     }
 
@@ -1169,9 +1174,9 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     }
     __syncthreads();
   }
-  if (bad_move) gatomic_add(a.status, 1u);
-  if (bad_dep) gatomic_add(a.status + 1, 1u);
-  if (bad_store) gatomic_add(a.status + 2, 1u);
+  if (bad_bits & 1) gatomic_add(a.status, 1u);
+  if (bad_bits & 2) gatomic_add(a.status + 1, 1u);
+  if (bad_bits & 4) gatomic_add(a.status + 2, 1u);
   if (a.counters) {
     if (n_gg) gatomic_add(a.counters + 0, (unsigned long long)n_gg);
     if (n_gd) gatomic_add(a.counters + 1, (unsigned long long)n_gd);
