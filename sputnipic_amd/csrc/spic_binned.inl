@@ -150,7 +150,7 @@ __device__ __forceinline__ void push_particle_once(const KGrid<T> &g, const KFie
                                               const KPush<T> &k,
                                               const Block<T, TX, TY, TZ> &blk, T &x,
                                               T &y, T &z, T &u, T &v, T &w,
-                                              bool &bad, unsigned int &n_lds,
+                                              int &bad, unsigned int &n_lds,
                                               unsigned int &n_glob) {
   using B = Block<T, TX, TY, TZ>;
   const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
@@ -300,7 +300,7 @@ template <typename T, bool STAGED, int TX, int TY, int TZ>
 __device__ __forceinline__ void push_particle(const KGrid<T> &g, const KField<T> &f,
                                               const KPush<T> &k,
                                               const Block<T, TX, TY, TZ> &blk, T &x, T &y,
-                                              T &z, T &u, T &v, T &w, bool &bad,
+                                              T &z, T &u, T &v, T &w, int &bad,
                                               unsigned int &n_lds, unsigned int &n_glob) {
 #pragma unroll 1
   for (int isub = 0; isub < k.nloop; ++isub)
@@ -330,7 +330,7 @@ __device__ __forceinline__ void push_fixed_once(const KGrid<float> &g, const KFi
                                            const KPush<float> &k, const FixedParams &fp,
                                            const Block<float, TX, TY, TZ> &blk,
                                            const int lim_b[3], int &X, int &Y, int &Z,
-                                           float &u, float &v, float &w, bool &bad,
+                                           float &u, float &v, float &w, int &bad,
                                            unsigned int &n_lds, unsigned int &n_glob) {
   using B = Block<float, TX, TY, TZ>;
   const int nyn = g.nyn, nzn = g.nzn, sxg = nyn * nzn;
@@ -493,7 +493,7 @@ __device__ __forceinline__ void push_fixed(const KGrid<float> &g, const KField<f
                                            const KPush<float> &k, const FixedParams &fp,
                                            const Block<float, TX, TY, TZ> &blk,
                                            const int lim_b[3], int &X, int &Y, int &Z,
-                                           float &u, float &v, float &w, bool &bad,
+                                           float &u, float &v, float &w, int &bad,
                                            unsigned int &n_lds, unsigned int &n_glob) {
 #pragma unroll 1
   for (int isub = 0; isub < k.nloop; ++isub)
@@ -922,7 +922,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       const uint32_t r = j * NT + tid;
       const bool active = r < nin;
       const int buf = (int)(j & 1);
-      bool bad_move = false, bad_dep = false, bad_store = false;  // this batch's
+      int bad_move = 0, bad_dep = 0, bad_store = 0;  // this batch's
       if (!PREFETCH) load_particle(r);
       T x = nxt[0], y = nxt[1], z = nxt[2], u = nxt[3], v = nxt[4], w = nxt[5], q = nxt[6];
       uint32_t pid = nxt_id;
@@ -1207,7 +1207,8 @@ __device__ __forceinline__ void binned_overflow_part(const BinnedArgs<T> &a, int
   none.fld = nullptr;
   none.xn = none.yn = none.zn = nullptr;
   none.c0x = none.c0y = none.c0z = 0;
-  bool bad_move = false, bad_dep = false, bad_store = false;
+  int bad_move = 0;
+  bool bad_dep = false, bad_store = false;
   unsigned int n_gg = 0, n_gd = 0, n_lg = 0;
   for (int s = 0; s < a.ns; ++s) {
     const BinnedSpecies<T> &sp = a.sp[s];
