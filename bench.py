@@ -63,14 +63,16 @@ def measured_traffic(prec: str):
 
 class EngineRunner:
     """A fresh engine of the 262M-particle Harris sheet with a given tile
-    shape and sort interval, for the autotuner (sputnipic_amd.tune)."""
+    shape, sort interval and particles per thread, for the autotuner
+    (sputnipic_amd.tune)."""
 
-    def __init__(self, prec: str, tile, sort_every: int):
+    def __init__(self, prec: str, tile, sort_every: int, ppt: int = 1):
         import sputnipic_amd as S
         self.grd = S.Grid(*GRID, *BOX, prec)
         gem = S.GemParams()
         self.field = S.gem_fields(self.grd, gem)
-        self.eng = S.Engine(self.grd, NS, tile=tile, sort_every=sort_every, set_moments=True)
+        self.eng = S.Engine(self.grd, NS, tile=tile, sort_every=sort_every, set_moments=True,
+                            particles_per_thread=ppt)
         for s, sp in enumerate(S.harris_species(PPC)):
             p = S.gem_particles(self.grd, gem, sp, s, seed=0)
             self.eng.load(s, p)
@@ -92,20 +94,21 @@ class EngineRunner:
 
 
 def tune_form(prec: str) -> dict:
-    """Tile shape and sort interval of the Harris-sheet engine, chosen by the
-    autotuner on the workload itself (each candidate a fresh engine, 1
-    warm-up and 10 timed phases)."""
+    """Tile shape, sort interval and (FP32 form) particles per thread of the
+    Harris-sheet engine, chosen by the autotuner on the workload itself
+    (each candidate a fresh engine, 1 warm-up and 10 timed phases)."""
     from sputnipic_amd.tune import tune_engine
     t0 = time.perf_counter()
-    res = tune_engine(lambda tile, se: EngineRunner(prec, tile, se), phases=10, warmup=1)
+    res = tune_engine(lambda tile, se, ppt=1: EngineRunner(prec, tile, se, ppt), phases=10,
+                      warmup=1, ppt_choices=(1, 2) if prec == "f32" else None)
     log(f"{prec}: tuned in {time.perf_counter() - t0:.1f} s: " + ", ".join(
-        f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}: {t.ms_per_phase:.3f} ms"
+        f"{'x'.join(map(str, t.tile))}/sort {t.sort_every}/ppt {t.ppt}: {t.ms_per_phase:.3f} ms"
         for t in res.trials))
     return res.as_dict()
 
 
 def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
-             tile, order: str = "load", overflow_pct: int = 0) -> dict:
+             tile, order: str = "load", overflow_pct: int = 0, ppt: int = 1) -> dict:
     import ctypes as C
 
     import sputnipic_amd as S
@@ -138,7 +141,7 @@ def run_form(prec: str, steps: int, warmup: int, engine: str, sort_every: int,
         # particles (0: all of them, a phase can never run out)
         eng = S.Engine(grd, NS, tile=tile, sort_every=0, set_moments=True,
                        order=S.SPIC_ORDER_ENGINE if order == "engine" else S.SPIC_ORDER_LOAD,
-                       overflow_pct=overflow_pct)
+                       overflow_pct=overflow_pct, particles_per_thread=ppt)
         t0 = time.perf_counter()
         for s, p in enumerate(parts):
             eng.load(s, p, stream)
@@ -984,6 +987,9 @@ def main() -> int:
     ap.add_argument("--engine", choices=("tiled", "plain"), default="tiled")
     ap.add_argument("--sort-every", type=int, default=0)
     ap.add_argument("--tile", default="4x4x4")
+    ap.add_argument("--ppt", type=int, choices=(1, 2), default=1,
+                    help="particles per thread and batch of the FP32 form's phase "
+                         "(with --no-tune; the autotuner otherwise chooses)")
     ap.add_argument("--order", choices=("load", "engine"), default="load",
                     help="engine particle order of the timed forms: load (a load index per "
                          "particle, store in load order; default) or engine (no index)")
@@ -1118,24 +1124,28 @@ def main() -> int:
     for prec in args.forms.split(","):
         tile = tuple(int(t) for t in args.tile.split("x"))
         sort_every = args.sort_every
+        ppt = args.ppt if prec == "f32" else 1
         if args.engine == "tiled" and not args.no_tune:
             with _StdoutToStderr():
                 tuning[prec] = tune_form(prec)
             tile = tuple(int(t) for t in tuning[prec]["tile"].split("x"))
             sort_every = tuning[prec]["sort_every"]
-        settings[prec] = (tile, sort_every)
+            ppt = tuning[prec]["particles_per_thread"]
+        settings[prec] = (tile, sort_every, ppt)
         forms[prec] = run_form(prec, args.steps, args.warmup, args.engine, sort_every,
-                               tile, args.order)
+                               tile, args.order, ppt=ppt)
         forms[prec]["tile"] = "x".join(map(str, tile))
+        forms[prec]["particles_per_thread"] = ppt
     # the same forms without the per-particle load index and with an overflow
     # region of LEAN_OVERFLOW_PCT percent (the smallest engine; reported beside
     # the headline, which keeps the default engine)
     forms_engine_order = {}
     if args.engine == "tiled" and args.order == "load" and not args.no_engine_order:
         for prec in args.forms.split(","):
-            tile, sort_every = settings[prec]
+            tile, sort_every, ppt = settings[prec]
             forms_engine_order[prec] = run_form(prec, args.steps, args.warmup, args.engine,
-                                                sort_every, tile, "engine", LEAN_OVERFLOW_PCT)
+                                                sort_every, tile, "engine", LEAN_OVERFLOW_PCT,
+                                                ppt=ppt)
     primary = "f64" if "f64" in forms else list(forms)[0]
     f = forms[primary]
     cpu = refleg = None
@@ -1175,6 +1185,7 @@ def main() -> int:
             "cycles_timed": args.steps,
             "tile": f.get("tile"),
             "sort_every": f.get("sort_every"),
+            "particles_per_thread": f.get("particles_per_thread"),
             "settings_from": "autotuner" if tuning else "command line",
         },
         "roofline": {

@@ -325,6 +325,14 @@ typedef struct spic_engine_opts {
                                  fit their segments and the region reports
                                  SPIC_ENOMEM (its output is then invalid:
                                  load the species again) */
+  int particles_per_thread;   /* particles each thread pushes and bins per
+                                 batch of the phase kernel before the
+                                 batch's bucket step: 0 or 1 (default), or
+                                 2 (FP32 engines with fixed-point positions;
+                                 the buckets double, two workgroups per CU
+                                 instead of three).  Trajectories do not
+                                 depend on it.  Batches (spic_batch_create)
+                                 and spic_engine_deposit use one */
 } spic_engine_opts;
 
 typedef struct spic_engine_stats {

@@ -624,19 +624,22 @@ class Engine:
     def __init__(self, grd: Grid, ns: int, tile=(4, 4, 4), sort_every: int = 0,
                  lds_partials: int = 0, slack_pct: int = 0, set_moments: bool = False,
                  sort_mode: int = 0, positions: int = 0, order: int = 0,
-                 overflow_pct: int = 0):
+                 overflow_pct: int = 0, particles_per_thread: int = 0):
         """positions: 0 (default: fixed point for an FP32 engine created in
         production arithmetic, else float), SPIC_POS_FLOAT or
         SPIC_POS_FIXED.  order: 0 / SPIC_ORDER_LOAD (store in load order) or
         SPIC_ORDER_ENGINE (no per-particle index; store in the engine's
-        order).  overflow_pct: 0 (default, n overflow slots) or 1..100."""
+        order).  overflow_pct: 0 (default, n overflow slots) or 1..100.
+        particles_per_thread: 0 / 1 (default) or 2 (FP32 fixed point): the
+        particles each thread pushes and bins per batch of the phase."""
         from ._capi import spic_engine_opts
         self.grd = grd
         self.ns = ns
         p = C.c_void_p()
         opts = spic_engine_opts(tile[0], tile[1], tile[2], sort_every, lds_partials,
                                 slack_pct, int(bool(set_moments)), int(sort_mode),
-                                int(positions), int(order), int(overflow_pct))
+                                int(positions), int(order), int(overflow_pct),
+                                int(particles_per_thread))
         g = grd.c_struct()
         check(lib().spic_engine_create(C.byref(p), C.byref(g), ns, C.byref(opts)),
               "spic_engine_create")

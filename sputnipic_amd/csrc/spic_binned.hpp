@@ -132,6 +132,8 @@ struct BinnedArgs {
   int n_ovf_blocks;        // workgroups for the overflow regions
   int push_off;            // 1: deposit only (interpP2G); no push, no field read
   int fixed;               // 1: positions in fixed point (FP32 engines), see above
+  int ppt;                 // particles per thread and batch of the phase kernel: 1, or
+                           // 2 (FP32 fixed point, single engines)
   FixedParams fx;
   BinnedSpecies<T> sp[kMaxSpecies];
   // status words: [0] mover iterate out of the grid, [1] deposit cell out

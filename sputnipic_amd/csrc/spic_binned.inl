@@ -784,14 +784,18 @@ __device__ __forceinline__ int fx_deposit_cell(int X, int k, int n, bool &bad) {
 // positions (FP32, production arithmetic): the x y z words of the particle
 // storage, the output buffer and the list ring hold the int32 coordinates,
 // the buckets the particle's offsets within its deposit cell.
-template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false,
+          int PPT = 1>
 __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const int tile) {
   static_assert(!FX || (sizeof(T) == 4 && !kStrictB), "fixed point: FP32 production only");
+  static_assert(PPT == 1 || PPT == 2, "one or two particles per thread and batch");
   using Blk = Block<T, TX, TY, TZ>;
   constexpr int NC = TX * TY * TZ;  // cells per tile
   constexpr int NT = 4 * NC;        // threads: four per cell
   constexpr int BX = Blk::BX, BY = Blk::BY, BZ = Blk::BZ, NB = Blk::NB;
-  constexpr int CAP = kBucketCap;
+  // PPT: particles each thread pushes and bins per batch before the bucket
+  // step, which then sums PPT x NT particles; the buckets grow with it
+  constexpr int CAP = kBucketCap * PPT;
   // bucket stride per cell: odd, so that the 16 cells a wave reads in the
   // bucket step fall on distinct LDS banks
   constexpr int CS = CAP | 1;
@@ -891,7 +895,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
 #pragma unroll
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
-    const uint32_t nbatches = (nin + NT - 1) / NT;
+    const uint32_t nbatches = (nin + NT * PPT - 1) / (NT * PPT);
     // this thread's particle of batch j (x y z u v w q and its id).  With
     // PREFETCH batch j + 1 is loaded while batch j is processed: FP32, which
     // has the registers for it at three waves per SIMD, and the deposit
@@ -919,14 +923,18 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     };
     if (PREFETCH) load_particle(tid);
     for (uint32_t j = 0; j < nbatches; ++j) {
-      const uint32_t r = j * NT + tid;
-      const bool active = r < nin;
       const int buf = (int)(j & 1);
       int bad_move = 0, bad_dep = 0, bad_store = 0;  // this batch's
+#pragma unroll 1
+      for (int pp = 0; pp < PPT; ++pp) {
+      // this thread's particle pp of batch j, and whether one follows
+      const uint32_t step = j * (uint32_t)PPT + (uint32_t)pp;
+      const uint32_t r = step * NT + tid;
+      const bool active = r < nin;
       if (!PREFETCH) load_particle(r);
       T x = nxt[0], y = nxt[1], z = nxt[2], u = nxt[3], v = nxt[4], w = nxt[5], q = nxt[6];
       uint32_t pid = nxt_id;
-      if (PREFETCH && j + 1 < nbatches) load_particle(r + NT);
+      if (PREFETCH && step + 1 < nbatches * (uint32_t)PPT) load_particle(r + NT);
       int ix = 2, iy = 2, iz = 2, dest = -1;
       bool in_tile = false;  // the deposit cell is one of the tile's cells
       if (active) {
@@ -1048,6 +1056,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
           }
         }
       }
+      }  // particle pp of the batch
 
       // bucket step: each cell's four threads sum the cell's bucket
       __syncthreads();
@@ -1187,12 +1196,15 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
 
 
 // PUSH false: the deposit alone (spic_engine_deposit), in its own
-// instances so that the phase kernels carry no test for it.
-template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false>
+// instances so that the phase kernels carry no test for it.  PPT 2 (two
+// particles per thread and batch, FP32 fixed point only): its buckets take
+// the LDS of a third workgroup, so it asks for two waves per SIMD.
+template <typename T, typename A, int TX, int TY, int TZ, bool PUSH = true, bool FX = false,
+          int PPT = 1>
 __global__ __launch_bounds__(4 * TX * TY * TZ)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+__attribute__((amdgpu_waves_per_eu(PPT == 2 ? 2 : sizeof(T) == 4 ? 3 : 2)))
 void k_binned_phase(BinnedArgs<T> a) {
-  binned_phase_tile<T, A, TX, TY, TZ, PUSH, FX>(a, blockIdx.x);
+  binned_phase_tile<T, A, TX, TY, TZ, PUSH, FX, PPT>(a, blockIdx.x);
 }
 
 // The particles of every species' input overflow region: global gathers
@@ -1378,14 +1390,30 @@ template hipError_t launch_binned_phase_queue<float>(const BinnedArgs<float> *,
 
 template <typename T, typename A, bool PUSH, bool FX>
 hipError_t launch_binned_shape(const BinnedArgs<T> &a, hipStream_t s) {
+  // two particles per thread and batch: the FP32 fixed-point phase only
+  constexpr bool kPpt2 = sizeof(T) == 4 && FX && PUSH;
+  const bool ppt2 = PUSH && a.ppt == 2;  // the deposit alone keeps one
+  if (ppt2 && !kPpt2) return hipErrorInvalidValue;
   if (a.tx == 4 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(256),
-                       0, s, a);
+    if constexpr (kPpt2) {
+      if (ppt2)
+        hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH, FX, 2>), dim3(a.ntiles),
+                           dim3(256), 0, s, a);
+    }
+    if (!ppt2)
+      hipLaunchKernelGGL((k_binned_phase<T, A, 4, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(256),
+                         0, s, a);
     hipLaunchKernelGGL((k_binned_overflow<T, 4, 4, 4, PUSH, FX>), dim3(a.n_ovf_blocks),
                        dim3(256), 0, s, a);
   } else if (a.tx == 8 && a.ty == 4 && a.tz == 4) {
-    hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(512),
-                       0, s, a);
+    if constexpr (kPpt2) {
+      if (ppt2)
+        hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH, FX, 2>), dim3(a.ntiles),
+                           dim3(512), 0, s, a);
+    }
+    if (!ppt2)
+      hipLaunchKernelGGL((k_binned_phase<T, A, 8, 4, 4, PUSH, FX>), dim3(a.ntiles), dim3(512),
+                         0, s, a);
     hipLaunchKernelGGL((k_binned_overflow<T, 8, 4, 4, PUSH, FX>), dim3(a.n_ovf_blocks),
                        dim3(256), 0, s, a);
   } else {

@@ -30,6 +30,7 @@ struct spic_engine {
   int set_moments = 0;             // phases write the moments (else add)
   int keep_ids = 1;                // a load index per particle (store in load order)
   int ovf_pct = 100;               // overflow slots per buffer, percent of n
+  int ppt = 1;                     // particles per thread and batch of the phase
   int64_t phases = 0, sorts = 0;
   uint64_t gen = 0;                // bumped by every load (storage may move)
   size_t bytes = 0;
@@ -226,6 +227,7 @@ int fill_args(const spic_engine *e, const spic_field *field, const spic_moments 
   a.ntiles = e->L.ntiles;
   a.tx = e->L.tx; a.ty = e->L.ty; a.tz = e->L.tz;
   a.lds_fp64 = e->lds_fp64;
+  a.ppt = e->ppt;
   a.fixed = e->L.fixed;
   a.fx = e->L.fx;
   a.n_ovf_blocks = kOverflowBlocks;
@@ -368,7 +370,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (ns < 1 || ns > spic::kMaxSpecies)
     return fail(SPIC_EINVAL, "engine species count must be 1..%d", spic::kMaxSpecies);
   int tx = 4, ty = 4, tz = 4, sort_every = 0, lds_fp64 = 1, slack_pct = 12, set_mom = 0,
-      sort_mode = SPIC_SORT_TILE, positions = 0, keep_ids = 1, ovf_pct = 100;
+      sort_mode = SPIC_SORT_TILE, positions = 0, keep_ids = 1, ovf_pct = 100, ppt = 1;
   if (opts) {
     if (opts->tile_x || opts->tile_y || opts->tile_z) {
       tx = opts->tile_x; ty = opts->tile_y; tz = opts->tile_z;
@@ -396,6 +398,9 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
     if (opts->overflow_pct < 0 || opts->overflow_pct > 100)
       return fail(SPIC_EINVAL, "overflow_pct must be 0..100");
     if (opts->overflow_pct) ovf_pct = opts->overflow_pct;
+    if (opts->particles_per_thread < 0 || opts->particles_per_thread > 2)
+      return fail(SPIC_EINVAL, "particles_per_thread must be 0, 1 or 2");
+    if (opts->particles_per_thread) ppt = opts->particles_per_thread;
   }
   if (positions == 0)
     positions = (grd->prec == SPIC_F32 && !spic::strict_mode()) ? SPIC_POS_FIXED
@@ -403,6 +408,9 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   if (positions == SPIC_POS_FIXED && grd->prec != SPIC_F32)
     return fail(SPIC_EUNSUPPORTED,
                 "fixed-point positions are for FP32 engines (an FP64 length is finer)");
+  if (ppt == 2 && positions != SPIC_POS_FIXED)
+    return fail(SPIC_EUNSUPPORTED,
+                "two particles per thread are built for FP32 fixed-point engines only");
   // the phase kernel packs a deposit cell into 12 + 10 + 10 bits
   if (grd->nxn > 4096 || grd->nyn > 1024 || grd->nzn > 1024)
     return fail(SPIC_EUNSUPPORTED,
@@ -423,6 +431,7 @@ int spic_engine_create(spic_engine **out, const spic_grid *grd, int ns,
   e->set_moments = set_mom;
   e->keep_ids = keep_ids;
   e->ovf_pct = ovf_pct;
+  e->ppt = ppt;
   auto &L = e->L;
   L.ncx = grd->nxc - 2; L.ncy = grd->nyc - 2; L.ncz = grd->nzc - 2;
   L.tx = tx; L.ty = ty; L.tz = tz;
@@ -1032,6 +1041,9 @@ int spic_batch_create(spic_batch **out, const spic_batch_member *members, int n,
     for (int is = 0; is < e->ns; ++is)
       if (e->sp[is].loaded && (rc = spic::check_moments(&m.moments[is]))) return rc;
     if ((rc = check_subcycles(e))) return rc;
+    if (e->ppt != 1)
+      return fail(SPIC_EUNSUPPORTED,
+                  "batch member %d: batches run one particle per thread and batch", i);
   }
   spic_batch *b = new (std::nothrow) spic_batch();
   if (!b) return fail(SPIC_ENOMEM, "batch allocation");

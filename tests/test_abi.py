@@ -159,3 +159,21 @@ def test_missing_library_fails_loudly(monkeypatch):
         _capi.lib()
     with pytest.raises(RuntimeError, match="libspic.so not built"):
         S.device_count()
+
+
+def test_engine_particles_per_thread_is_validated_before_any_device_work():
+    """particles_per_thread: 0, 1 or 2, and 2 only for FP32 engines with
+    fixed-point positions; both refusals come before any device call (the
+    grid's node tables are fake pointers here)."""
+    L = _capi.lib()
+    g, f, s, p = _fake_args()
+    eng = C.c_void_p()
+    o = _capi.spic_engine_opts()
+    o.particles_per_thread = 3
+    assert L.spic_engine_create(C.byref(eng), C.byref(g), 1, C.byref(o)) == _capi.SPIC_EINVAL
+    assert b"particles_per_thread" in L.spic_last_error()
+    o.particles_per_thread = 2  # an FP64 grid stores float positions
+    assert L.spic_engine_create(C.byref(eng), C.byref(g), 1,
+                                C.byref(o)) == _capi.SPIC_EUNSUPPORTED
+    assert b"fixed-point" in L.spic_last_error()
+    assert not eng.value
