@@ -363,3 +363,53 @@ def test_engine_autotuner_times_every_candidate_and_picks_the_fastest():
         for k in ("u", "v", "w"):
             got, want = last[0].out[s][k], st.parts[s][k]
             assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), (s, k)
+
+
+def test_engine_autotuner_times_particles_per_thread():
+    """With ppt_choices the tuner builds each (tile, sort interval) with one
+    and with two particles per thread (an FP32 fixed-point engine), passes
+    the choice to the builder, records it per trial and returns the fastest
+    trial's; both choices advance the same particles, bit for bit."""
+    from sputnipic_amd.tune import tune_engine
+    S.set_strict_fp(False)  # fixed-point positions: production arithmetic
+    st = _oracle(INSTANCES[0], "f32")
+    c = st.cfg
+    built, outs = [], {}
+
+    class Runner:
+        def __init__(self, tile, se, ppt):
+            self.key = (tile, se, ppt)
+            built.append(self.key)
+            self.grd = S.Grid(c.nxc, c.nyc, c.nzc, c.Lx, c.Ly, c.Lz, "f32")
+            self.field = S.EMField(self.grd, {n: st.fields[n].astype(np.float32)
+                                              for n in O.FIELD_NAMES})
+            self.parts = [S.Particles(p, c.qom[s], "f32", c.NiterMover)
+                          for s, p in enumerate(st.parts)]
+            self.ids = [S.InterpDensSpecies(self.grd) for _ in range(c.ns)]
+            self.param = S.Parameters(dt=c.dt, c=c.c, NiterMover=c.NiterMover,
+                                      PERIODICX=c.PERIODICX, PERIODICY=c.PERIODICY,
+                                      PERIODICZ=c.PERIODICZ)
+            self.eng = S.Engine(self.grd, c.ns, tile=tile, sort_every=se, set_moments=True,
+                                particles_per_thread=ppt)
+            for s, p in enumerate(self.parts):
+                self.eng.load(s, p)
+
+        def phase(self, stream):
+            self.eng.phase(self.field, self.ids, self.param, stream)
+
+        def close(self):
+            for s, p in enumerate(self.parts):
+                self.eng.store(s, p)
+            outs[self.key] = [p.download() for p in self.parts]
+            self.eng.close()
+
+    res = tune_engine(lambda tile, se, ppt: Runner(tile, se, ppt), tiles=((4, 4, 4),),
+                      sort_choices=(0,), phases=2, warmup=1, ppt_choices=(1, 2))
+    assert built == [((4, 4, 4), 0, 1), ((4, 4, 4), 0, 2)]
+    assert [t.ppt for t in res.trials] == [1, 2]
+    best = min(res.trials, key=lambda t: t.ms_per_phase)
+    assert res.ppt == best.ppt and res.as_dict()["particles_per_thread"] == best.ppt
+    a, b = outs[((4, 4, 4), 0, 1)], outs[((4, 4, 4), 0, 2)]
+    for s in range(c.ns):
+        for k in O.PARTICLE_NAMES:
+            assert np.array_equal(a[s][k], b[s][k]), (s, k)
