@@ -896,15 +896,19 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
     for (int k = 0; k < 20; ++k) acc[k] = T(0);
 
     const uint32_t nbatches = (nin + NT * PPT - 1) / (NT * PPT);
-    // this thread's particle of batch j (x y z u v w q and its id).  With
-    // PREFETCH batch j + 1 is loaded while batch j is processed: FP32, which
-    // has the registers for it at three waves per SIMD, and the deposit
-    // alone (8.2 -> 7.3 ms per FP64 deposit of the 262M-particle sheet).
-    // The FP64 phase kernel loads each batch as it starts: its push needs
-    // all 256 VGPRs, and loading batch j + 1 during the push (spills) or
-    // after it (in flight over the bucket and list steps) measured 2-5%
-    // slower.
+    // this thread's particle of batch j (x y z u v w q and its id), loaded
+    // one batch ahead.  PREFETCH: batch j + 1 is loaded while batch j is
+    // processed -- FP32, which has the registers for it at three waves per
+    // SIMD, and the deposit alone (8.2 -> 7.3 ms per FP64 deposit of the
+    // 262M-particle sheet).  LATE (the FP64 phase, whose push needs all 256
+    // VGPRs; loading during the push spills): batch j + 1 is loaded once
+    // batch j is pushed and binned, before its particles are stored, so that
+    // the loads cannot take the registers those stores still read (a load
+    // into a register a pending store reads waits for the store) and are in
+    // flight over the stores, the bucket and the list steps: 2% faster than
+    // loading at the start of each batch, and than loading after the stores.
     constexpr bool PREFETCH = sizeof(T) == 4 || !PUSH;
+    constexpr bool LATE = !PREFETCH;
     T nxt[7];
     uint32_t nxt_id = 0;
     auto load_particle = [&](uint32_t r) {
@@ -921,7 +925,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
                                                    i * 4u);
       }
     };
-    if (PREFETCH) load_particle(tid);
+    if (PREFETCH || LATE) load_particle(tid);
     for (uint32_t j = 0; j < nbatches; ++j) {
       const int buf = (int)(j & 1);
       int bad_move = 0, bad_dep = 0, bad_store = 0;  // this batch's
@@ -931,7 +935,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
       const uint32_t step = j * (uint32_t)PPT + (uint32_t)pp;
       const uint32_t r = step * NT + tid;
       const bool active = r < nin;
-      if (!PREFETCH) load_particle(r);
+      if (!PREFETCH && !LATE) load_particle(r);
       T x = nxt[0], y = nxt[1], z = nxt[2], u = nxt[3], v = nxt[4], w = nxt[5], q = nxt[6];
       uint32_t pid = nxt_id;
       if (PREFETCH && step + 1 < nbatches * (uint32_t)PPT) load_particle(r + NT);
@@ -1040,6 +1044,7 @@ __device__ __forceinline__ void binned_phase_tile(const BinnedArgs<T> &a, const 
         }
       }
 
+      if (LATE && step + 1 < nbatches * (uint32_t)PPT) load_particle(r + NT);
       // the bucketed particles stay in the tile: their slots come from the
       // workgroup's reserved block, one LDS counter atomic per wave
       {
